@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark: exact FP32 bilateral filter, r=15, on an 8192x8192 u8 image.
+
+One step = one bf_exact launch over the whole image (67.1 Mpix, REF mode =
+the reference kernel's semantics) with input and output resident in HBM.
+Prints ONE JSON line (see the keys below).  Progress goes to stderr.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+
+Roofline: the kernel is FP32-VALU bound (not a contraction, so no MFMA;
+2 bytes/px of HBM traffic against 3849 FLOP/px).  Algorithmic work is
+4*T + 1 FLOP per pixel with T = (2r+1)^2 + 1 = 962 taps (g*E, f*p, t+=, s+=
+per tap, one divide).  `achieved` = that work per launch / the average
+launch time measured with HIP events on the launch stream; `peak` = the
+MI355X FP32 vector nameplate (157.3 TFLOP/s, which assumes packed FMA).
+
+cpu_baseline: the oracle (host transcription, OpenMP) on a band of rows of
+the same image, timed on this host's cores; the band is also compared
+bit-exactly with the GPU output.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Mpix/s r=15 on 8192² u8: exact FP32 vs BF16 surrogate+fallback (≤2 LSB)"
+PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector nameplate (packed FMA)
+PEAK_FP32_NOFMA_TFLOPS = 78.6  # packed mul/add, no FMA
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(img, it, gpu_out, seconds: float):
+    """Oracle (the host transcription) on a band of rows sized for about
+    `seconds` of work, all threads; returns the baseline dict and whether
+    the band matched the GPU output bit for bit."""
+    import numpy as np
+    import oracle
+    nthreads = min(os.cpu_count() or 1, 16)
+    H = img.shape[0]
+    y0 = (H // 2) // 4 * 4
+    # calibrate on 8 rows
+    t = time.perf_counter()
+    oracle.filter(img, it.radius, it.sigma_space, it.sigma_range, mode=oracle.REF,
+                  rows=(y0, y0 + 8), nthreads=nthreads)
+    dt = time.perf_counter() - t
+    rows = int(max(8, min(H - y0, seconds / max(dt, 1e-6) * 8)) // 4 * 4)
+    log(f"cpu baseline: {rows} rows x {img.shape[1]} px on {nthreads} threads")
+    t = time.perf_counter()
+    band = oracle.filter(img, it.radius, it.sigma_space, it.sigma_range, mode=oracle.REF,
+                         rows=(y0, y0 + rows), nthreads=nthreads)
+    dt = time.perf_counter() - t
+    px = band.size
+    match = bool(np.array_equal(band, gpu_out[y0:y0 + rows]))
+    # 1-thread rate on a quarter of the band's rows (at least 4)
+    r1 = max(4, rows // (4 * nthreads) // 4 * 4)
+    t = time.perf_counter()
+    oracle.filter(img, it.radius, it.sigma_space, it.sigma_range, mode=oracle.REF,
+                  rows=(y0, y0 + r1), nthreads=1)
+    dt1 = time.perf_counter() - t
+    base = {
+        "value": round(px / dt / 1e6, 3), "unit": "Mpix/s", "cores": nthreads, "kind": "port",
+        "sample": f"oracle REF r=15 on rows {y0}..{y0 + rows} of the 8192^2 image "
+                  f"({px} px, {dt:.1f} s)",
+        "one_thread_mpix_s": round(img.shape[1] * r1 / dt1 / 1e6, 4),
+        "nproc": os.cpu_count(),
+    }
+    return base, match, (y0, y0 + rows)
+
+
+def fp64_error(img, it, gpu_out, rows):
+    import numpy as np
+    import oracle
+    v = oracle.filter_f64(img, it.radius, it.sigma_space, it.sigma_range, mode=oracle.REF,
+                          rows=rows, nthreads=min(os.cpu_count() or 1, 16))
+    err = gpu_out[rows[0]:rows[1]].astype(np.float64) - v
+    return {"rows": list(rows), "mean": round(float(err.mean()), 4),
+            "mean_abs": round(float(np.abs(err).mean()), 4),
+            "max_abs": round(float(np.abs(err).max()), 4)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, "
+              f"WORLD_SIZE {world}); the filter shards by row bands with no exchange, "
+              "see DESIGN.md. Run with --gpus 1.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import opencuda_amd as oc
+    from opencuda_amd import workloads as wl
+
+    torch.cuda.set_device(0)
+    (it,) = wl.c3()
+    log("generating the 8192^2 workload image")
+    img = it.image()
+    H, W = img.shape
+    lut = oc.Lut.from_sigmas(it.sigma_space, it.sigma_range)
+    tin = torch.from_numpy(img).cuda()
+    tout = torch.zeros_like(tin)
+    vin, vout = oc.view(tin), oc.view(tout)
+    stream = torch.cuda.current_stream()
+
+    log(f"warmup x{args.warmup}")
+    for _ in range(args.warmup):
+        oc.exact(vin, vout, lut, it.radius, oc.MODE_REF, stream=stream)
+    torch.cuda.synchronize()
+
+    log(f"timing x{args.steps}")
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        oc.exact(vin, vout, lut, it.radius, oc.MODE_REF, stream=stream)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per_launch_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    span_ms = starts[0].elapsed_time(ends[-1])
+    ms_per_step = span_ms / args.steps
+    kern_ms = float(np.mean(per_launch_ms))
+    px = it.pixels
+    value = px / (ms_per_step / 1e3) / 1e6
+    T = (2 * it.radius + 1) ** 2 + 1
+    flop_px = 4 * T + 1
+    achieved = px * flop_px / (kern_ms / 1e3) / 1e12
+    log(f"exact REF: {ms_per_step:.3f} ms/step, {value:.0f} Mpix/s, {achieved:.2f} TFLOP/s "
+        f"(wall {wall:.3f} s)")
+    gpu_out = tout.cpu().numpy()
+
+    # STD mode (the surrogate's target) on the same image, for reference.
+    for _ in range(2):
+        oc.exact(vin, vout, lut, it.radius, oc.MODE_STD, stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    nstd = max(3, args.steps // 4)
+    for _ in range(nstd):
+        oc.exact(vin, vout, lut, it.radius, oc.MODE_STD, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    std_ms = e0.elapsed_time(e1) / nstd
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "exact_c3_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "c3: 8192x8192 u8 synthetic (1600 rects + 1600 discs, noise 12), "
+                        "full ROI, r=15, sigma_space=10, sigma_range=30, exact FP32 filter "
+                        "in REF mode (the reference kernel's semantics), 962 taps/px",
+            "path": "bf_exact",
+            "pixels": px,
+            "std_mode_mpix_s": round(px / (std_ms / 1e3) / 1e6, 2),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": traffic,
+            "frac_of_nofma_peak": round(achieved / PEAK_FP32_NOFMA_TFLOPS, 4),
+            "flop_per_px": flop_px,
+            "kernel_ms": round(kern_ms, 4),
+            "hbm_gbs_algorithmic": round(2 * px / (kern_ms / 1e3) / 1e9, 1),
+        },
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle)")
+        base, match, rows = cpu_baseline(img, it, gpu_out, args.cpu_seconds)
+        result["cpu_baseline"] = base
+        result["bit_exact_sample"] = match
+        result["fp64_error"] = fp64_error(img, it, gpu_out, (rows[0], rows[0] + 16))
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,553 @@
+// bf_abi.cpp -- C ABI of the exact and batched filter (include/opencuda_bf.h).
+//
+// Host-side bookkeeping: validation with the reference's status codes,
+// device tables, tile planning, launches.  Kernels are in bf_exact.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bf_internal.h"
+#include "opencuda_bf.h"
+
+namespace bfimpl {
+
+static thread_local std::string g_last_error;
+static std::atomic<int> g_next_lut_id{ 1 };
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char *what)
+{
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return kHipError;
+}
+
+int current_device()
+{
+    int d = -1;
+    if (hipGetDevice(&d) != hipSuccess) return -1;
+    return d;
+}
+
+// Per-(lut, ntaps) device tap arrays live in one buffer: entry k of the
+// variant with n real taps sits at variant_offset(n) + k.
+struct LutVariants {
+    std::unordered_map<int, int> offset_of;   // ntaps -> first entry
+};
+
+struct LutFull : Lut {
+    LutVariants var;
+};
+
+Lut *lut_from_handle(bf_lut h) { return reinterpret_cast<Lut *>(h); }
+
+static int ntaps_for(int radius, int mode)
+{
+    const int g = (2 * radius + 1) * (2 * radius + 1);
+    return mode == BF_MODE_REF ? g + 1 : g;
+}
+
+static int padded(int n) { return (n + kTapPad - 1) / kTapPad * kTapPad; }
+
+int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_base, int radius,
+             int mode, int config, int tile_begin, DevJob *job, int64_t *pixels)
+{
+    if (!in || !out || !lut_base || !job) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    const LutFull *lut = static_cast<const LutFull *>(lut_base);
+    if (mode != BF_MODE_REF && mode != BF_MODE_STD) {
+        set_error("mode must be BF_MODE_REF or BF_MODE_STD");
+        return kInvalidData;
+    }
+    if (radius <= 0 || radius > BF_MAX_RADIUS) {
+        set_error("radius must be in 1..20");
+        return kInvalidData;
+    }
+    if (!in->data || !out->data) {
+        set_error("image view has no data");
+        return kNullPointer;
+    }
+    if (in->width <= 0 || in->height <= 0 || out->width <= 0 || out->height <= 0 ||
+        in->pitch < in->width || out->pitch < out->width) {
+        set_error("empty image view or pitch smaller than width");
+        return kInvalidData;
+    }
+    const int nt = ntaps_for(radius, mode);
+    if (nt > lut->count) {
+        set_error("template holds " + std::to_string(lut->count) + " points, radius " +
+                  std::to_string(radius) + " needs " + std::to_string(nt) +
+                  (mode == BF_MODE_REF ? " (REF reads (2r+1)^2+1 taps)" : ""));
+        return kInvalidData;
+    }
+    auto it = lut->var.offset_of.find(nt);
+    if (it == lut->var.offset_of.end()) {
+        set_error("no tap variant for this radius");
+        return kInvalidData;
+    }
+    const int halo = halo_class(radius);
+    // Every tap must stay inside the staged halo.
+    for (int c = 0; c < nt; c++) {
+        const int dy = lut->offsets[2 * c], dx = lut->offsets[2 * c + 1];
+        if (dx < -halo || dx > halo || dy < -halo || dy > halo + (mode == BF_MODE_REF ? 2 : 0) ||
+            (mode == BF_MODE_STD && (dy < -radius || dy > radius || dx < -radius || dx > radius))) {
+            set_error("template tap outside the radius/halo the kernel stages");
+            return kInvalidData;
+        }
+    }
+    const TileConfig &tc = tile_config(config);
+    const int W = std::min(in->width, out->width);
+    const int H = std::min(in->height, out->height);
+    std::memset(job, 0, sizeof(*job));
+    job->in = in->data;
+    job->out = out->data;
+    job->in_pitch = in->pitch;
+    job->out_pitch = out->pitch;
+    job->in_w = in->width;
+    job->in_h = in->height;
+    job->W = W;
+    job->H = H;
+    job->taps = lut->d_taps + it->second;
+    job->rowg = lut->d_rowg;
+    job->range = lut->d_range;
+    job->ntaps_padded = padded(nt);
+    job->nrows = lut->rows_upto[nt];
+    job->null_row = job->nrows;
+    job->mode = mode;
+    job->lut_key = lut->id;
+    job->tiles_x = (W + tc.tw - 1) / tc.tw;
+    job->tile_begin = tile_begin;
+    if (pixels) *pixels = (int64_t)W * H;
+    return kOk;
+}
+
+static int tiles_of(const DevJob &j, int config)
+{
+    const TileConfig &tc = tile_config(config);
+    return j.tiles_x * ((j.H + 4 * tc.tgy - 1) / (4 * tc.tgy));
+}
+
+static bool views_overlap(const bf_image_view *a, const bf_image_view *b)
+{
+    const uint8_t *a0 = a->data, *a1 = a->data + (int64_t)(a->height - 1) * a->pitch + a->width;
+    const uint8_t *b0 = b->data, *b1 = b->data + (int64_t)(b->height - 1) * b->pitch + b->width;
+    return a0 < b1 && b0 < a1;
+}
+
+}  // namespace bfimpl
+
+using namespace bfimpl;
+
+extern "C" {
+
+const char *bf_last_error(void) { return g_last_error.c_str(); }
+
+int bf_version(void) { return 10000; }
+
+int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, const float *range,
+                  bf_lut *out)
+{
+    if (!offsets || !weights || !range || !out) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    if (count <= 0 || count > 65535) {
+        set_error("template point count must be 1..65535");
+        return kInvalidData;
+    }
+    for (int c = 0; c < count; c++) {
+        if (offsets[2 * c] < -127 || offsets[2 * c] > 127 || offsets[2 * c + 1] < -127 ||
+            offsets[2 * c + 1] > 127) {
+            set_error("template offset out of int8 range");
+            return kInvalidData;
+        }
+    }
+    auto *L = new (std::nothrow) LutFull();
+    if (!L) return kOutOfMem;
+    L->id = g_next_lut_id.fetch_add(1);
+    L->count = count;
+    L->offsets.assign(offsets, offsets + 2 * (size_t)count);
+    L->weights.assign(weights, weights + count);
+    L->range.assign(range, range + 256);
+    // Rows of the fused table: distinct weight bit patterns, in order of
+    // first appearance, so the first n taps use rows [0, rows_upto[n]).
+    std::unordered_map<uint32_t, int> row_of_bits;
+    std::vector<float> rowg;
+    L->row_of_tap.resize(count);
+    L->rows_upto.assign(count + 1, 0);
+    for (int c = 0; c < count; c++) {
+        uint32_t bits;
+        std::memcpy(&bits, &weights[c], 4);
+        auto it = row_of_bits.find(bits);
+        int row;
+        if (it == row_of_bits.end()) {
+            row = (int)rowg.size();
+            row_of_bits.emplace(bits, row);
+            rowg.push_back(weights[c]);
+        } else {
+            row = it->second;
+        }
+        L->row_of_tap[c] = row;
+        L->rows_upto[c + 1] = (int)rowg.size();
+    }
+    L->nrows_total = (int)rowg.size();
+    // Tap variants for every radius and mode that fits in the template.
+    std::vector<uint32_t> packed;
+    for (int mode = 0; mode < 2; mode++) {
+        for (int r = 1; r <= BF_MAX_RADIUS; r++) {
+            const int nt = ntaps_for(r, mode);
+            if (nt > count || L->var.offset_of.count(nt)) continue;
+            L->var.offset_of[nt] = (int)packed.size();
+            const int nrows = L->rows_upto[nt];
+            for (int c = 0; c < padded(nt); c++) {
+                if (c < nt)
+                    packed.push_back(pack_tap(offsets[2 * c], offsets[2 * c + 1], L->row_of_tap[c]));
+                else
+                    packed.push_back(pack_tap(0, 0, nrows));   // null tap: zero row
+            }
+        }
+    }
+    if (packed.empty()) packed.push_back(0);
+    L->device = current_device();
+    hipError_t e = hipMalloc(&L->d_taps, packed.size() * 4);
+    if (e == hipSuccess) e = hipMalloc(&L->d_rowg, std::max<size_t>(1, rowg.size()) * 4);
+    if (e == hipSuccess) e = hipMalloc(&L->d_range, 256 * 4);
+    if (e == hipSuccess) e = hipMemcpy(L->d_taps, packed.data(), packed.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !rowg.empty())
+        e = hipMemcpy(L->d_rowg, rowg.data(), rowg.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->d_range, range, 256 * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(L->d_taps);
+        hipFree(L->d_rowg);
+        hipFree(L->d_range);
+        delete L;
+        return hip_fail(e, "bf_lut_create");
+    }
+    *out = reinterpret_cast<bf_lut>(static_cast<Lut *>(L));
+    return kOk;
+}
+
+int bf_lut_destroy(bf_lut h)
+{
+    if (!h) return kNullPointer;
+    auto *L = static_cast<LutFull *>(lut_from_handle(h));
+    hipFree(L->d_taps);
+    hipFree(L->d_rowg);
+    hipFree(L->d_range);
+    delete L;
+    return kOk;
+}
+
+int bf_lut_rows(bf_lut h, int32_t ntaps)
+{
+    if (!h) return kNullPointer;
+    Lut *L = lut_from_handle(h);
+    if (ntaps < 0 || ntaps > L->count) return kInvalidData;
+    return L->rows_upto[ntaps];
+}
+
+int bf_tune_count(void) { return tile_config_count(); }
+
+int bf_tune_describe(int32_t config, int32_t *tile_w, int32_t *tile_h, int32_t *threads,
+                     int32_t *cols_per_lane)
+{
+    if (config < 0 || config >= tile_config_count()) return kInvalidData;
+    const TileConfig &c = tile_config(config);
+    if (tile_w) *tile_w = c.tw;
+    if (tile_h) *tile_h = 4 * c.tgy;
+    if (threads) *threads = c.threads;
+    if (cols_per_lane) *cols_per_lane = 1;
+    return kOk;
+}
+
+static int check_fits(int config, int radius, int nrows)
+{
+    const int halo = halo_class(radius);
+    if (lds_bytes(config, halo, nrows) > max_lds_bytes()) {
+        set_error("tile config " + std::to_string(config) + " needs " +
+                  std::to_string(lds_bytes(config, halo, nrows)) +
+                  " bytes of LDS at radius " + std::to_string(radius));
+        return kInvalidData;
+    }
+    return kOk;
+}
+
+// Pick a configuration that fits in LDS, preferring `want`.
+static int fit_config(int want, int radius, int nrows)
+{
+    const int halo = halo_class(radius);
+    if (want >= 0 && lds_bytes(want, halo, nrows) <= max_lds_bytes()) return want;
+    for (int c = 0; c < tile_config_count(); c++)
+        if (lds_bytes(c, halo, nrows) <= max_lds_bytes()) return c;
+    return -1;
+}
+
+int bf_exact_config(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int32_t radius,
+                    int32_t mode, int32_t config, void *stream)
+{
+    if (config < 0 || config >= tile_config_count()) {
+        set_error("bad tile config");
+        return kInvalidData;
+    }
+    if (!in || !out || !lut) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    if (in->data && out->data && in->width > 0 && out->width > 0 && views_overlap(in, out)) {
+        set_error("input and output views overlap");
+        return kInvalidData;
+    }
+    DevJob job;
+    int rc = make_job(in, out, lut_from_handle(lut), radius, mode, config, 0, &job, nullptr);
+    if (rc) return rc;
+    rc = check_fits(config, radius, job.nrows);
+    if (rc) return rc;
+    return launch_exact(config, halo_class(radius), job, nullptr, 1, tiles_of(job, config),
+                        job.nrows, nullptr, (hipStream_t)stream);
+}
+
+int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int32_t radius,
+             int32_t mode, void *stream)
+{
+    if (!lut) {
+        set_error("null lut");
+        return kNullPointer;
+    }
+    Lut *L = lut_from_handle(lut);
+    const int nt = ntaps_for(radius < 1 ? 1 : (radius > BF_MAX_RADIUS ? BF_MAX_RADIUS : radius), mode);
+    const int nrows = nt <= L->count ? L->rows_upto[nt] : 0;
+    int config = fit_config(default_config(radius, mode), radius < 1 ? 1 : radius, nrows);
+    if (config < 0) {
+        set_error("no tile configuration fits the weight table in LDS");
+        return kInvalidData;
+    }
+    return bf_exact_config(in, out, lut, radius, mode, config, stream);
+}
+
+int bf_exact_inplace(const bf_image_view *img, bf_lut lut, int32_t radius, int32_t mode,
+                     int32_t repeat, unsigned char *work, void *stream)
+{
+    if (!img || !lut) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    if (repeat <= 0) {
+        set_error("repeat must be >= 1");
+        return kInvalidData;
+    }
+    if (!img->data || img->width <= 0 || img->height <= 0) {
+        set_error("empty image view");
+        return kInvalidData;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    unsigned char *tmp = work;
+    const size_t bytes = (size_t)img->width * img->height;
+    if (!tmp) BF_HIP(hipMallocAsync(reinterpret_cast<void **>(&tmp), bytes, s));
+    bf_image_view scratch{ tmp, img->width, img->height, img->width };
+    int rc = kOk;
+    // Each pass: snapshot the image into scratch, filter scratch -> image.
+    for (int i = 0; i < repeat && rc == kOk; i++) {
+        hipError_t e = hipMemcpy2DAsync(tmp, img->width, img->data, img->pitch, img->width,
+                                        img->height, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipMemcpy2DAsync");
+            break;
+        }
+        rc = bf_exact(&scratch, img, lut, radius, mode, stream);
+    }
+    if (!work) {
+        hipError_t e = hipFreeAsync(tmp, s);
+        if (rc == kOk && e != hipSuccess) rc = hip_fail(e, "hipFreeAsync");
+    }
+    return rc;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------- batches
+
+namespace bfimpl {
+
+struct Batch {
+    std::vector<DevJob> jobs;      // sorted, tile_begin assigned
+    DevJob *d_jobs = nullptr;
+    int32_t *d_counter = nullptr;
+    int ntiles = 0;
+    int config = 0;
+    int halo = 4;
+    int max_rows = 0;
+    int64_t pixels = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = -1;
+};
+
+}  // namespace bfimpl
+
+extern "C" {
+
+int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch *out)
+{
+    if (!jobs || !out) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    if (njobs <= 0) {
+        set_error("empty batch");
+        return kInvalidData;
+    }
+    int rmax = 0;
+    for (int i = 0; i < njobs; i++) rmax = std::max(rmax, (int)jobs[i].radius);
+    if (rmax < 1 || rmax > BF_MAX_RADIUS) {
+        set_error("radius must be in 1..20");
+        return kInvalidData;
+    }
+    const int halo = halo_class(rmax);
+    // Validate and size every job with a provisional config.
+    int cfg = config >= 0 ? config : default_config(rmax, jobs[0].mode);
+    if (cfg >= tile_config_count()) {
+        set_error("bad tile config");
+        return kInvalidData;
+    }
+    std::vector<DevJob> dj(njobs);
+    std::vector<int64_t> cost(njobs);
+    int max_rows = 0;
+    int64_t pixels = 0;
+    for (int i = 0; i < njobs; i++) {
+        int64_t px = 0;
+        if (!jobs[i].lut) {
+            set_error("job " + std::to_string(i) + " has no lut");
+            return kNullPointer;
+        }
+        if (views_overlap(&jobs[i].in, &jobs[i].out)) {
+            set_error("job " + std::to_string(i) + ": input and output views overlap");
+            return kInvalidData;
+        }
+        int rc = make_job(&jobs[i].in, &jobs[i].out, lut_from_handle(jobs[i].lut), jobs[i].radius,
+                          jobs[i].mode, cfg, 0, &dj[i], &px);
+        if (rc) {
+            set_error("job " + std::to_string(i) + ": " + g_last_error);
+            return rc;
+        }
+        max_rows = std::max(max_rows, dj[i].nrows);
+        cost[i] = (int64_t)dj[i].ntaps_padded;
+        pixels += px;
+    }
+    if (config < 0) {
+        int c = fit_config(cfg, rmax, max_rows);
+        if (c < 0) {
+            set_error("no tile configuration fits the weight tables in LDS");
+            return kInvalidData;
+        }
+        cfg = c;
+    } else if (lds_bytes(cfg, halo, max_rows) > max_lds_bytes()) {
+        set_error("tile config does not fit the weight tables in LDS");
+        return kInvalidData;
+    }
+    // Longest (per-tile cost) first, grouped by table so a workgroup walking
+    // the queue rebuilds its LDS table rarely.
+    std::vector<int> order(njobs);
+    for (int i = 0; i < njobs; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        if (cost[a] != cost[b]) return cost[a] > cost[b];
+        return dj[a].lut_key < dj[b].lut_key;
+    });
+    auto *B = new (std::nothrow) Batch();
+    if (!B) return kOutOfMem;
+    B->config = cfg;
+    B->halo = halo;
+    B->max_rows = max_rows;
+    B->pixels = pixels;
+    B->device = current_device();
+    const TileConfig &tc = tile_config(cfg);
+    int next = 0;
+    for (int k = 0; k < njobs; k++) {
+        DevJob j = dj[order[k]];
+        j.tiles_x = (j.W + tc.tw - 1) / tc.tw;
+        j.tile_begin = next;
+        next += tiles_of(j, cfg);
+        B->jobs.push_back(j);
+    }
+    B->ntiles = next;
+    hipError_t e = hipMalloc(&B->d_jobs, sizeof(DevJob) * B->jobs.size());
+    if (e == hipSuccess) e = hipMalloc(&B->d_counter, sizeof(int32_t));
+    if (e == hipSuccess)
+        e = hipMemcpy(B->d_jobs, B->jobs.data(), sizeof(DevJob) * B->jobs.size(),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(B->d_jobs);
+        hipFree(B->d_counter);
+        delete B;
+        return hip_fail(e, "bf_batch_create");
+    }
+    *out = reinterpret_cast<bf_batch>(B);
+    return kOk;
+}
+
+int bf_batch_destroy(bf_batch h)
+{
+    if (!h) return kNullPointer;
+    auto *B = reinterpret_cast<Batch *>(h);
+    if (B->exec) hipGraphExecDestroy(B->exec);
+    if (B->graph) hipGraphDestroy(B->graph);
+    hipFree(B->d_jobs);
+    hipFree(B->d_counter);
+    delete B;
+    return kOk;
+}
+
+int bf_batch_run(bf_batch h, void *stream)
+{
+    if (!h) return kNullPointer;
+    auto *B = reinterpret_cast<Batch *>(h);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = launch_reset_counter(B->d_counter, s);
+    if (rc) return rc;
+    return launch_exact(B->config, B->halo, B->jobs[0], B->d_jobs, (int)B->jobs.size(), B->ntiles,
+                        B->max_rows, B->d_counter, s);
+}
+
+int bf_batch_run_graph(bf_batch h, void *stream)
+{
+    if (!h) return kNullPointer;
+    auto *B = reinterpret_cast<Batch *>(h);
+    hipStream_t s = (hipStream_t)stream;
+    if (!B->exec) {
+        // Capture on a private stream so the caller's stream state is untouched.
+        hipStream_t cap;
+        BF_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        int rc = kOk;
+        if (e == hipSuccess) rc = bf_batch_run(h, cap);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(cap, &g);
+        hipStreamDestroy(cap);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+        B->graph = g;
+        BF_HIP(hipGraphInstantiate(&B->exec, g, nullptr, nullptr, 0));
+    }
+    BF_HIP(hipGraphLaunch(B->exec, s));
+    return kOk;
+}
+
+int64_t bf_batch_pixels(bf_batch h)
+{
+    if (!h) return kNullPointer;
+    return reinterpret_cast<Batch *>(h)->pixels;
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// bf_internal.h -- shared declarations of the HIP implementation (not a
+// public header; the C ABI lives in include/opencuda_bf.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "opencuda_bf.h"
+
+namespace bfimpl {
+
+// Status codes (same values as ErrorCode.h).
+constexpr int kOk = 0;
+constexpr int kInvalidData = -1;
+constexpr int kNullPointer = -2;
+constexpr int kOutOfMem = -6;
+constexpr int kHipError = -7;
+constexpr int kUnmatchImg = -8;
+
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define BF_HIP(call)                                              \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return ::bfimpl::hip_fail(e_, #call); \
+    } while (0)
+
+// One tap of the spatial template as the kernels read it:
+//   bits  0..7  dy (int8, row offset), bits 8..15 dx (int8, column offset),
+//   bits 16..31 row of the fused weight table W[row][d] = g_row * E[d].
+inline uint32_t pack_tap(int dy, int dx, int row)
+{
+    return (uint32_t)(uint8_t)(int8_t)dy | ((uint32_t)(uint8_t)(int8_t)dx << 8) |
+           ((uint32_t)row << 16);
+}
+
+// Taps are padded to a multiple of this with "null taps" whose weight row
+// is all zeros: t + 0*p == t and s + 0 == s bit for bit, so padding never
+// changes a result.
+constexpr int kTapPad = 4;
+
+// Device-resident filter tables.
+struct Lut {
+    int id = 0;                  // unique per table set (LDS reuse key)
+    int count = 0;               // template points
+    std::vector<int32_t> offsets;    // host copy, 2*count
+    std::vector<float> weights;      // host copy, count
+    std::vector<float> range;        // host copy, 256
+    std::vector<int> row_of_tap;     // count
+    std::vector<int> rows_upto;      // rows_upto[n] = distinct weights in taps [0,n)
+    int nrows_total = 0;
+    int device = -1;
+    uint32_t *d_taps = nullptr;      // count + kTapPad entries (tail = null taps)
+    float *d_rowg = nullptr;         // nrows_total
+    float *d_range = nullptr;        // 256
+};
+
+// Job as the exact kernel reads it (one per image in a batch).
+struct alignas(16) DevJob {
+    const uint8_t *in;
+    uint8_t *out;
+    int64_t in_pitch;
+    int64_t out_pitch;
+    int32_t in_w, in_h;      // clamp extent (input ROI size)
+    int32_t W, H;            // filtered size
+    const uint32_t *taps;
+    const float *rowg;
+    const float *range;
+    int32_t ntaps_padded;    // multiple of kTapPad; tail taps are null
+    int32_t nrows;           // rows of W used by the real taps; row nrows is the zero row
+    int32_t mode;
+    int32_t lut_key;
+    int32_t tiles_x;
+    int32_t tile_begin;      // first global tile index of this job
+    int32_t null_row;        // == nrows
+    int32_t pad0;
+};
+
+struct TileConfig {
+    int tw;        // tile width = lanes along x
+    int tgy;       // 4-row groups along y; tile height = 4*tgy
+    int threads;   // tw * tgy
+};
+
+int tile_config_count();
+const TileConfig &tile_config(int id);
+int halo_class(int radius);   // compile-time halo (>= radius) used for a radius
+int default_config(int radius, int mode);
+
+// LDS bytes one workgroup needs.
+size_t lds_bytes(int config, int halo, int nrows);
+size_t max_lds_bytes();
+
+// Launch the exact kernel over `ntiles` tiles of the jobs.
+//   jobs_dev: device job table (njobs entries), or nullptr when njobs == 1
+//             (then job0 is passed by value);
+//   counter:  device int for the work queue, or nullptr for a static
+//             grid-stride schedule.
+int launch_exact(int config, int halo, const DevJob &job0, const DevJob *jobs_dev,
+                 int njobs, int ntiles, int max_rows, int *counter, hipStream_t s);
+int launch_reset_counter(int *counter, hipStream_t s);
+
+// Build a DevJob for one view pair (validates; fills tiles_x for config).
+int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut,
+             int radius, int mode, int config, int tile_begin, DevJob *job,
+             int64_t *pixels);
+
+Lut *lut_from_handle(bf_lut h);
+int current_device();
+
+// Surrogate helpers used by bf_sur_filter.
+int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_dev_max,
+                const int32_t *count_dev, int config, int halo, hipStream_t s);
+
+}  // namespace bfimpl
