@@ -105,6 +105,12 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_b
         }
     }
     const TileConfig &tc = tile_config(config);
+    const int nrows = lut->rows_upto[nt];
+    const bool sep = lds_bytes(config, halo, nrows + 1) > max_lds_bytes();
+    if (lds_bytes(config, halo, sep ? 1 : nrows + 1) > max_lds_bytes()) {
+        set_error("tile config " + std::to_string(config) + " does not fit in LDS");
+        return kInvalidData;
+    }
     const int W = std::min(in->width, out->width);
     const int H = std::min(in->height, out->height);
     std::memset(job, 0, sizeof(*job));
@@ -117,17 +123,26 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_b
     job->W = W;
     job->H = H;
     job->taps = lut->d_taps + it->second;
+    job->gtap = lut->d_gtap + it->second;
     job->rowg = lut->d_rowg;
     job->range = lut->d_range;
     job->ntaps_padded = padded(nt);
-    job->nrows = lut->rows_upto[nt];
-    job->null_row = job->nrows;
+    job->nrows = nrows;
+    job->sep = sep ? 1 : 0;
+    job->lut_rows = sep ? 1 : nrows + 1;
     job->mode = mode;
-    job->lut_key = lut->id;
+    job->lut_key = 2 * lut->id + (sep ? 1 : 0);
     job->tiles_x = (W + tc.tw - 1) / tc.tw;
     job->tile_begin = tile_begin;
     if (pixels) *pixels = (int64_t)W * H;
     return kOk;
+}
+
+int fused_rows(const Lut *lut, int radius, int mode)
+{
+    if (radius < 1 || radius > BF_MAX_RADIUS) return -1;
+    const int nt = ntaps_for(radius, mode);
+    return nt <= lut->count ? lut->rows_upto[nt] : -1;
 }
 
 static int tiles_of(const DevJob &j, int config)
@@ -202,6 +217,7 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
     L->nrows_total = (int)rowg.size();
     // Tap variants for every radius and mode that fits in the template.
     std::vector<uint32_t> packed;
+    std::vector<float> gtap;
     for (int mode = 0; mode < 2; mode++) {
         for (int r = 1; r <= BF_MAX_RADIUS; r++) {
             const int nt = ntaps_for(r, mode);
@@ -209,24 +225,33 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
             L->var.offset_of[nt] = (int)packed.size();
             const int nrows = L->rows_upto[nt];
             for (int c = 0; c < padded(nt); c++) {
-                if (c < nt)
+                if (c < nt) {
                     packed.push_back(pack_tap(offsets[2 * c], offsets[2 * c + 1], L->row_of_tap[c]));
-                else
-                    packed.push_back(pack_tap(0, 0, nrows));   // null tap: zero row
+                    gtap.push_back(weights[c]);
+                } else {
+                    packed.push_back(pack_tap(0, 0, nrows));   // null tap: zero row / g = 0
+                    gtap.push_back(0.0f);
+                }
             }
         }
     }
-    if (packed.empty()) packed.push_back(0);
+    if (packed.empty()) {
+        packed.push_back(0);
+        gtap.push_back(0.0f);
+    }
     L->device = current_device();
     hipError_t e = hipMalloc(&L->d_taps, packed.size() * 4);
+    if (e == hipSuccess) e = hipMalloc(&L->d_gtap, gtap.size() * 4);
     if (e == hipSuccess) e = hipMalloc(&L->d_rowg, std::max<size_t>(1, rowg.size()) * 4);
     if (e == hipSuccess) e = hipMalloc(&L->d_range, 256 * 4);
     if (e == hipSuccess) e = hipMemcpy(L->d_taps, packed.data(), packed.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(L->d_gtap, gtap.data(), gtap.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && !rowg.empty())
         e = hipMemcpy(L->d_rowg, rowg.data(), rowg.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(L->d_range, range, 256 * 4, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         hipFree(L->d_taps);
+        hipFree(L->d_gtap);
         hipFree(L->d_rowg);
         hipFree(L->d_range);
         delete L;
@@ -241,6 +266,7 @@ int bf_lut_destroy(bf_lut h)
     if (!h) return kNullPointer;
     auto *L = static_cast<LutFull *>(lut_from_handle(h));
     hipFree(L->d_taps);
+    hipFree(L->d_gtap);
     hipFree(L->d_rowg);
     hipFree(L->d_range);
     delete L;
@@ -269,26 +295,16 @@ int bf_tune_describe(int32_t config, int32_t *tile_w, int32_t *tile_h, int32_t *
     return kOk;
 }
 
-static int check_fits(int config, int radius, int nrows)
-{
-    const int halo = halo_class(radius);
-    if (lds_bytes(config, halo, nrows) > max_lds_bytes()) {
-        set_error("tile config " + std::to_string(config) + " needs " +
-                  std::to_string(lds_bytes(config, halo, nrows)) +
-                  " bytes of LDS at radius " + std::to_string(radius));
-        return kInvalidData;
-    }
-    return kOk;
-}
-
-// Pick a configuration that fits in LDS, preferring `want`.
+// Pick a tile configuration for (radius, fused-table rows): `want` if the
+// fused table fits with it, else any configuration where it fits, else
+// `want` in separate-table mode.
 static int fit_config(int want, int radius, int nrows)
 {
     const int halo = halo_class(radius);
-    if (want >= 0 && lds_bytes(want, halo, nrows) <= max_lds_bytes()) return want;
+    if (lds_bytes(want, halo, nrows + 1) <= max_lds_bytes()) return want;
     for (int c = 0; c < tile_config_count(); c++)
-        if (lds_bytes(c, halo, nrows) <= max_lds_bytes()) return c;
-    return -1;
+        if (lds_bytes(c, halo, nrows + 1) <= max_lds_bytes()) return c;
+    return want;
 }
 
 int bf_exact_config(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int32_t radius,
@@ -302,17 +318,16 @@ int bf_exact_config(const bf_image_view *in, const bf_image_view *out, bf_lut lu
         set_error("null argument");
         return kNullPointer;
     }
-    if (in->data && out->data && in->width > 0 && out->width > 0 && views_overlap(in, out)) {
+    if (in->data && out->data && in->width > 0 && out->width > 0 && in->height > 0 &&
+        out->height > 0 && views_overlap(in, out)) {
         set_error("input and output views overlap");
         return kInvalidData;
     }
     DevJob job;
     int rc = make_job(in, out, lut_from_handle(lut), radius, mode, config, 0, &job, nullptr);
     if (rc) return rc;
-    rc = check_fits(config, radius, job.nrows);
-    if (rc) return rc;
     return launch_exact(config, halo_class(radius), job, nullptr, 1, tiles_of(job, config),
-                        job.nrows, nullptr, (hipStream_t)stream);
+                        job.lut_rows, nullptr, (hipStream_t)stream);
 }
 
 int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int32_t radius,
@@ -322,14 +337,9 @@ int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int3
         set_error("null lut");
         return kNullPointer;
     }
-    Lut *L = lut_from_handle(lut);
-    const int nt = ntaps_for(radius < 1 ? 1 : (radius > BF_MAX_RADIUS ? BF_MAX_RADIUS : radius), mode);
-    const int nrows = nt <= L->count ? L->rows_upto[nt] : 0;
-    int config = fit_config(default_config(radius, mode), radius < 1 ? 1 : radius, nrows);
-    if (config < 0) {
-        set_error("no tile configuration fits the weight table in LDS");
-        return kInvalidData;
-    }
+    const int rows = fused_rows(lut_from_handle(lut), radius, mode);
+    const int config = rows < 0 ? default_config(1, mode)
+                                : fit_config(default_config(radius, mode), radius, rows);
     return bf_exact_config(in, out, lut, radius, mode, config, stream);
 }
 
@@ -420,7 +430,7 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
     }
     std::vector<DevJob> dj(njobs);
     std::vector<int64_t> cost(njobs);
-    int max_rows = 0;
+    int max_rows = 0, max_lut_rows = 0;
     int64_t pixels = 0;
     for (int i = 0; i < njobs; i++) {
         int64_t px = 0;
@@ -443,13 +453,18 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
         pixels += px;
     }
     if (config < 0) {
-        int c = fit_config(cfg, rmax, max_rows);
-        if (c < 0) {
-            set_error("no tile configuration fits the weight tables in LDS");
-            return kInvalidData;
+        const int c = fit_config(cfg, rmax, max_rows);
+        if (c != cfg) {   // re-plan the jobs with the configuration that fits
+            cfg = c;
+            for (int i = 0; i < njobs; i++) {
+                int rc = make_job(&jobs[i].in, &jobs[i].out, lut_from_handle(jobs[i].lut),
+                                  jobs[i].radius, jobs[i].mode, cfg, 0, &dj[i], nullptr);
+                if (rc) return rc;
+            }
         }
-        cfg = c;
-    } else if (lds_bytes(cfg, halo, max_rows) > max_lds_bytes()) {
+    }
+    for (int i = 0; i < njobs; i++) max_lut_rows = std::max(max_lut_rows, dj[i].lut_rows);
+    if (lds_bytes(cfg, halo, max_lut_rows) > max_lds_bytes()) {
         set_error("tile config does not fit the weight tables in LDS");
         return kInvalidData;
     }
@@ -465,7 +480,7 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
     if (!B) return kOutOfMem;
     B->config = cfg;
     B->halo = halo;
-    B->max_rows = max_rows;
+    B->max_rows = max_lut_rows;
     B->pixels = pixels;
     B->device = current_device();
     const TileConfig &tc = tile_config(cfg);

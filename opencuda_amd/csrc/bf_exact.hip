@@ -171,9 +171,15 @@ __device__ __forceinline__ void load_tile(const DevJob &job, int X0, int Y0, int
 }
 
 // Fused weight table: W[row][d] = rowg[row] * range[d]; row `nrows` = 0.
+// Separate-table mode: just E[d].
 template <int NT>
 __device__ __forceinline__ void build_lut(const DevJob &job, uint32_t lut_base, int tid)
 {
+    if (job.sep) {
+        for (int d = tid; d < 256; d += NT)
+            lds_store_u32(lut_base + (uint32_t)d * 4, __float_as_uint(job.range[d]));
+        return;
+    }
     const int n = (job.nrows + 1) * 256;
     for (int e = tid; e < n; e += NT) {
         const int row = e >> 8, d = e & 255;
@@ -183,7 +189,7 @@ __device__ __forceinline__ void build_lut(const DevJob &job, uint32_t lut_base, 
     }
 }
 
-template <int TW, int TGY, int HALO>
+template <int TW, int TGY, int HALO, bool SEP>
 __device__ __forceinline__ void tile_ref(const DevJob &job, int X0, int Y0, int lx, int gy)
 {
     using G = Geo<TW, TGY, HALO>;
@@ -195,6 +201,7 @@ __device__ __forceinline__ void tile_ref(const DevJob &job, int X0, int Y0, int 
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
     const BF_CONST uint32_t *taps = as_const(job.taps);
+    const BF_CONST float *gtap = as_const(job.gtap);
     const int nt = job.ntaps_padded;
     for (int c = 0; c < nt; c += kTapPad) {
 #pragma unroll
@@ -202,13 +209,17 @@ __device__ __forceinline__ void tile_ref(const DevJob &job, int X0, int Y0, int 
             const uint32_t info = taps[c + u];
             const int dy = (int)(int8_t)(info & 0xFFu);
             const int dx = (int)(int8_t)((info >> 8) & 0xFFu);
-            const uint32_t wrow = LUT_BASE + ((info >> 16) << 10);
+            const uint32_t wrow = SEP ? LUT_BASE : LUT_BASE + ((info >> 16) << 10);
+            const float g = SEP ? gtap[c + u] : 1.0f;
             const uint32_t p = lds_u32(lane + (uint32_t)((dy * G::TCOLS + dx) * 4));
             const float pf = (float)(p & 0xFFu);
-            const float f0 = lds_f32(__builtin_amdgcn_sad_u8(p, c0, wrow));
-            const float f1 = lds_f32(__builtin_amdgcn_sad_u8(p, c1, wrow));
-            const float f2 = lds_f32(__builtin_amdgcn_sad_u8(p, c2, wrow));
-            const float f3 = lds_f32(__builtin_amdgcn_sad_u8(p, c3, wrow));
+            float f0 = lds_f32(__builtin_amdgcn_sad_u8(p, c0, wrow));
+            float f1 = lds_f32(__builtin_amdgcn_sad_u8(p, c1, wrow));
+            float f2 = lds_f32(__builtin_amdgcn_sad_u8(p, c2, wrow));
+            float f3 = lds_f32(__builtin_amdgcn_sad_u8(p, c3, wrow));
+            if (SEP) {   // factor = g[col] * E[d] (BilateralFilter.cu:125), one rounding
+                f0 = g * f0; f1 = g * f1; f2 = g * f2; f3 = g * f3;
+            }
             t0 = t0 + f0 * pf; s0 = s0 + f0;
             t1 = t1 + f1 * pf; s1 = s1 + f1;
             t2 = t2 + f2 * pf; s2 = s2 + f2;
@@ -225,7 +236,7 @@ __device__ __forceinline__ void tile_ref(const DevJob &job, int X0, int Y0, int 
     }
 }
 
-template <int TW, int TGY, int HALO>
+template <int TW, int TGY, int HALO, bool SEP>
 __device__ __forceinline__ void tile_std(const DevJob &job, int X0, int Y0, int lx, int gy)
 {
     using G = Geo<TW, TGY, HALO>;
@@ -237,6 +248,7 @@ __device__ __forceinline__ void tile_std(const DevJob &job, int X0, int Y0, int 
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
     const BF_CONST uint32_t *taps = as_const(job.taps);
+    const BF_CONST float *gtap = as_const(job.gtap);
     const int nt = job.ntaps_padded;
     for (int c = 0; c < nt; c += kTapPad) {
 #pragma unroll
@@ -244,14 +256,18 @@ __device__ __forceinline__ void tile_std(const DevJob &job, int X0, int Y0, int 
             const uint32_t info = taps[c + u];
             const int dy = (int)(int8_t)(info & 0xFFu);
             const int dx = (int)(int8_t)((info >> 8) & 0xFFu);
-            const uint32_t wrow = LUT_BASE + ((info >> 16) << 10);
+            const uint32_t wrow = SEP ? LUT_BASE : LUT_BASE + ((info >> 16) << 10);
+            const float g = SEP ? gtap[c + u] : 1.0f;
             const uint32_t a = lane + (uint32_t)((dy * G::TCOLS + dx) * 4);
             const uint32_t p0 = lds_u32(a), p1 = lds_u32(a + ROWB);
             const uint32_t p2 = lds_u32(a + 2 * ROWB), p3 = lds_u32(a + 3 * ROWB);
-            const float f0 = lds_f32(__builtin_amdgcn_sad_u8(p0, c0, wrow));
-            const float f1 = lds_f32(__builtin_amdgcn_sad_u8(p1, c1, wrow));
-            const float f2 = lds_f32(__builtin_amdgcn_sad_u8(p2, c2, wrow));
-            const float f3 = lds_f32(__builtin_amdgcn_sad_u8(p3, c3, wrow));
+            float f0 = lds_f32(__builtin_amdgcn_sad_u8(p0, c0, wrow));
+            float f1 = lds_f32(__builtin_amdgcn_sad_u8(p1, c1, wrow));
+            float f2 = lds_f32(__builtin_amdgcn_sad_u8(p2, c2, wrow));
+            float f3 = lds_f32(__builtin_amdgcn_sad_u8(p3, c3, wrow));
+            if (SEP) {
+                f0 = g * f0; f1 = g * f1; f2 = g * f2; f3 = g * f3;
+            }
             t0 = t0 + f0 * (float)(p0 & 0xFFu); s0 = s0 + f0;
             t1 = t1 + f1 * (float)(p1 & 0xFFu); s1 = s1 + f1;
             t2 = t2 + f2 * (float)(p2 & 0xFFu); s2 = s2 + f2;
@@ -313,10 +329,17 @@ bf_exact_kernel(KArgs args)
         }
         load_tile<TW, TGY, HALO>(job, X0, Y0, tid);
         __syncthreads();
-        if (job.mode == kModeRef)
-            tile_ref<TW, TGY, HALO>(job, X0, Y0, lx, gy);
-        else
-            tile_std<TW, TGY, HALO>(job, X0, Y0, lx, gy);
+        if (job.sep) {
+            if (job.mode == kModeRef)
+                tile_ref<TW, TGY, HALO, true>(job, X0, Y0, lx, gy);
+            else
+                tile_std<TW, TGY, HALO, true>(job, X0, Y0, lx, gy);
+        } else {
+            if (job.mode == kModeRef)
+                tile_ref<TW, TGY, HALO, false>(job, X0, Y0, lx, gy);
+            else
+                tile_std<TW, TGY, HALO, false>(job, X0, Y0, lx, gy);
+        }
     }
 }
 
@@ -336,10 +359,10 @@ constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kHalos[] = { 4, 8, 12, 16, 20 };
 
 template <int TW, int TGY, int HALO>
-int launch_one(const KArgs &a, int max_rows, hipStream_t s)
+int launch_one(const KArgs &a, int max_lut_rows, hipStream_t s)
 {
     using G = Geo<TW, TGY, HALO>;
-    const size_t lds = kTileOff + (size_t)G::TILE_BYTES + (size_t)(max_rows + 1) * 1024;
+    const size_t lds = kTileOff + (size_t)G::TILE_BYTES + (size_t)max_lut_rows * 1024;
     auto kern = bf_exact_kernel<TW, TGY, HALO>;
     static int configured = 0;   // per instantiation
     if (!configured) {
@@ -365,14 +388,14 @@ int launch_one(const KArgs &a, int max_rows, hipStream_t s)
 }
 
 template <int TW, int TGY>
-int launch_halo(int halo, const KArgs &a, int max_rows, hipStream_t s)
+int launch_halo(int halo, const KArgs &a, int max_lut_rows, hipStream_t s)
 {
     switch (halo) {
-    case 4: return launch_one<TW, TGY, 4>(a, max_rows, s);
-    case 8: return launch_one<TW, TGY, 8>(a, max_rows, s);
-    case 12: return launch_one<TW, TGY, 12>(a, max_rows, s);
-    case 16: return launch_one<TW, TGY, 16>(a, max_rows, s);
-    case 20: return launch_one<TW, TGY, 20>(a, max_rows, s);
+    case 4: return launch_one<TW, TGY, 4>(a, max_lut_rows, s);
+    case 8: return launch_one<TW, TGY, 8>(a, max_lut_rows, s);
+    case 12: return launch_one<TW, TGY, 12>(a, max_lut_rows, s);
+    case 16: return launch_one<TW, TGY, 16>(a, max_lut_rows, s);
+    case 20: return launch_one<TW, TGY, 20>(a, max_lut_rows, s);
     default: set_error("bad halo class"); return kInvalidData;
     }
 }
@@ -396,17 +419,17 @@ int default_config(int radius, int mode)
     return radius >= 9 ? 0 : 1;
 }
 
-size_t lds_bytes(int config, int halo, int nrows)
+size_t lds_bytes(int config, int halo, int lut_rows)
 {
     const TileConfig &c = kConfigs[config];
     return kTileOff + (size_t)(c.tw + 2 * halo) * (size_t)(4 * c.tgy + 2 * halo) * 4 +
-           (size_t)(nrows + 1) * 1024;
+           (size_t)lut_rows * 1024;
 }
 
 size_t max_lds_bytes() { return 160 * 1024; }
 
 int launch_exact(int config, int halo, const DevJob &job0, const DevJob *jobs_dev, int njobs,
-                 int ntiles, int max_rows, int *counter, hipStream_t s)
+                 int ntiles, int max_lut_rows, int *counter, hipStream_t s)
 {
     KArgs a{};
     a.job0 = job0;
@@ -417,12 +440,12 @@ int launch_exact(int config, int halo, const DevJob &job0, const DevJob *jobs_de
     a.tile_count = nullptr;
     a.counter = counter;
     switch (config) {
-    case 0: return launch_halo<64, 16>(halo, a, max_rows, s);
-    case 1: return launch_halo<64, 8>(halo, a, max_rows, s);
-    case 2: return launch_halo<32, 8>(halo, a, max_rows, s);
-    case 3: return launch_halo<64, 4>(halo, a, max_rows, s);
-    case 4: return launch_halo<128, 4>(halo, a, max_rows, s);
-    case 5: return launch_halo<128, 8>(halo, a, max_rows, s);
+    case 0: return launch_halo<64, 16>(halo, a, max_lut_rows, s);
+    case 1: return launch_halo<64, 8>(halo, a, max_lut_rows, s);
+    case 2: return launch_halo<32, 8>(halo, a, max_lut_rows, s);
+    case 3: return launch_halo<64, 4>(halo, a, max_lut_rows, s);
+    case 4: return launch_halo<128, 4>(halo, a, max_lut_rows, s);
+    case 5: return launch_halo<128, 8>(halo, a, max_lut_rows, s);
     default: set_error("bad tile config"); return kInvalidData;
     }
 }
@@ -439,8 +462,8 @@ int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_max,
     a.tile_count = count_dev;
     a.counter = nullptr;
     switch (config) {
-    case 0: return launch_halo<64, 16>(halo, a, job.nrows, s);
-    case 1: return launch_halo<64, 8>(halo, a, job.nrows, s);
+    case 0: return launch_halo<64, 16>(halo, a, job.lut_rows, s);
+    case 1: return launch_halo<64, 8>(halo, a, job.lut_rows, s);
     default: set_error("fallback needs a 64-wide tile config"); return kInvalidData;
     }
 }

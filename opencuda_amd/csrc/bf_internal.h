@@ -54,7 +54,8 @@ struct Lut {
     std::vector<int> rows_upto;      // rows_upto[n] = distinct weights in taps [0,n)
     int nrows_total = 0;
     int device = -1;
-    uint32_t *d_taps = nullptr;      // count + kTapPad entries (tail = null taps)
+    uint32_t *d_taps = nullptr;      // tap variants (see bf_lut_create)
+    float *d_gtap = nullptr;         // spatial weight per entry of d_taps (0 for null taps)
     float *d_rowg = nullptr;         // nrows_total
     float *d_range = nullptr;        // 256
 };
@@ -70,14 +71,15 @@ struct alignas(16) DevJob {
     const uint32_t *taps;
     const float *rowg;
     const float *range;
+    const float *gtap;       // per-tap spatial weight (separate-table mode)
     int32_t ntaps_padded;    // multiple of kTapPad; tail taps are null
     int32_t nrows;           // rows of W used by the real taps; row nrows is the zero row
     int32_t mode;
-    int32_t lut_key;
+    int32_t lut_key;         // identity of the LDS table contents
     int32_t tiles_x;
     int32_t tile_begin;      // first global tile index of this job
-    int32_t null_row;        // == nrows
-    int32_t pad0;
+    int32_t sep;             // 1: LDS holds E only, g*E formed per tap (large radii)
+    int32_t lut_rows;        // 1 KB rows of the LDS table: nrows + 1, or 1 when sep
 };
 
 struct TileConfig {
@@ -91,8 +93,8 @@ const TileConfig &tile_config(int id);
 int halo_class(int radius);   // compile-time halo (>= radius) used for a radius
 int default_config(int radius, int mode);
 
-// LDS bytes one workgroup needs.
-size_t lds_bytes(int config, int halo, int nrows);
+// LDS bytes one workgroup needs for a table of lut_rows 1 KB rows.
+size_t lds_bytes(int config, int halo, int lut_rows);
 size_t max_lds_bytes();
 
 // Launch the exact kernel over `ntiles` tiles of the jobs.
@@ -101,13 +103,17 @@ size_t max_lds_bytes();
 //   counter:  device int for the work queue, or nullptr for a static
 //             grid-stride schedule.
 int launch_exact(int config, int halo, const DevJob &job0, const DevJob *jobs_dev,
-                 int njobs, int ntiles, int max_rows, int *counter, hipStream_t s);
+                 int njobs, int ntiles, int max_lut_rows, int *counter, hipStream_t s);
 int launch_reset_counter(int *counter, hipStream_t s);
 
-// Build a DevJob for one view pair (validates; fills tiles_x for config).
+// Build a DevJob for one view pair (validates; fills tiles_x for config;
+// chooses the separate-table mode when the fused table does not fit).
 int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut,
              int radius, int mode, int config, int tile_begin, DevJob *job,
              int64_t *pixels);
+// Fused-table rows needed by (lut, radius, mode), or -1 if the radius
+// exceeds the template.
+int fused_rows(const Lut *lut, int radius, int mode);
 
 Lut *lut_from_handle(bf_lut h);
 int current_device();
