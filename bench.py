@@ -94,6 +94,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one plain 256 MiB device-to-device copy (torch copy_): "
+                         "the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -109,6 +112,14 @@ def main() -> int:
     from opencuda_amd import workloads as wl
 
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     (it,) = wl.c3()
     log("generating the 8192^2 workload image")
     img = it.image()

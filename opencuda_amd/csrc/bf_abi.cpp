@@ -58,7 +58,7 @@ static int ntaps_for(int radius, int mode)
 static int padded(int n) { return (n + kTapPad - 1) / kTapPad * kTapPad; }
 
 int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_base, int radius,
-             int mode, int config, int tile_begin, DevJob *job, int64_t *pixels)
+             int mode, int config, int halo, int tile_begin, DevJob *job, int64_t *pixels)
 {
     if (!in || !out || !lut_base || !job) {
         set_error("null argument");
@@ -94,7 +94,10 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_b
         set_error("no tap variant for this radius");
         return kInvalidData;
     }
-    const int halo = halo_class(radius);
+    if (halo < halo_class(radius)) {
+        set_error("halo smaller than the radius needs");
+        return kInvalidData;
+    }
     // Every tap must stay inside the staged halo.
     for (int c = 0; c < nt; c++) {
         const int dy = lut->offsets[2 * c], dx = lut->offsets[2 * c + 1];
@@ -131,7 +134,10 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_b
     job->sep = sep ? 1 : 0;
     job->lut_rows = sep ? 1 : nrows + 1;
     job->mode = mode;
-    job->lut_key = 2 * lut->id + (sep ? 1 : 0);
+    // LDS table identity: the same tables with another row count (REF has
+    // the extra tap's row, and the zero row moves) or mode is another table.
+    // (A fused table that fits in LDS has < 160 rows; a separate table is E only.)
+    job->lut_key = ((lut->id & 0x7FFFF) << 12) | ((sep ? 0 : nrows) << 1) | (sep ? 1 : 0);
     job->tiles_x = (W + tc.tw - 1) / tc.tw;
     job->tile_begin = tile_begin;
     if (pixels) *pixels = (int64_t)W * H;
@@ -324,7 +330,8 @@ int bf_exact_config(const bf_image_view *in, const bf_image_view *out, bf_lut lu
         return kInvalidData;
     }
     DevJob job;
-    int rc = make_job(in, out, lut_from_handle(lut), radius, mode, config, 0, &job, nullptr);
+    int rc = make_job(in, out, lut_from_handle(lut), radius, mode, config, halo_class(radius), 0,
+                      &job, nullptr);
     if (rc) return rc;
     return launch_exact(config, halo_class(radius), job, nullptr, 1, tiles_of(job, config),
                         job.lut_rows, nullptr, (hipStream_t)stream);
@@ -443,7 +450,7 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
             return kInvalidData;
         }
         int rc = make_job(&jobs[i].in, &jobs[i].out, lut_from_handle(jobs[i].lut), jobs[i].radius,
-                          jobs[i].mode, cfg, 0, &dj[i], &px);
+                          jobs[i].mode, cfg, halo, 0, &dj[i], &px);
         if (rc) {
             set_error("job " + std::to_string(i) + ": " + g_last_error);
             return rc;
@@ -458,7 +465,7 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
             cfg = c;
             for (int i = 0; i < njobs; i++) {
                 int rc = make_job(&jobs[i].in, &jobs[i].out, lut_from_handle(jobs[i].lut),
-                                  jobs[i].radius, jobs[i].mode, cfg, 0, &dj[i], nullptr);
+                                  jobs[i].radius, jobs[i].mode, cfg, halo, 0, &dj[i], nullptr);
                 if (rc) return rc;
             }
         }

@@ -413,10 +413,12 @@ int halo_class(int radius)
     return -1;
 }
 
+// Measured on MI355X (8192-wide image): 64x64 tiles win at r >= 13 in both
+// modes, 64x32 tiles below (REF r = 3..9, STD r = 7..9).
 int default_config(int radius, int mode)
 {
     (void)mode;
-    return radius >= 9 ? 0 : 1;
+    return radius >= 13 ? 0 : 1;
 }
 
 size_t lds_bytes(int config, int halo, int lut_rows)

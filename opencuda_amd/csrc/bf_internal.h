@@ -109,7 +109,7 @@ int launch_reset_counter(int *counter, hipStream_t s);
 // Build a DevJob for one view pair (validates; fills tiles_x for config;
 // chooses the separate-table mode when the fused table does not fit).
 int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut,
-             int radius, int mode, int config, int tile_begin, DevJob *job,
+             int radius, int mode, int config, int halo, int tile_begin, DevJob *job,
              int64_t *pixels);
 // Fused-table rows needed by (lut, radius, mode), or -1 if the radius
 // exceeds the template.
