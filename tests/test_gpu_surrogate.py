@@ -1,0 +1,192 @@
+"""The learned surrogate's MFMA kernels on the GPU.
+
+* forward and gradients against an independent float64 CPU implementation
+  (torch autograd on the CPU) with the same BF16 rounding points: weights,
+  input channels and post-ReLU activations rounded to BF16;
+* training lowers the loss and is deterministic;
+* the check-and-fallback path: tolerance -1 makes every tile exact
+  (bit-exact to bf_exact), tolerance 255 leaves the pure surrogate output;
+* the trained network against the method without it (identity / untrained
+  network): what fraction of pixels it gets within 2 LSB of the exact
+  filter, and how many 64x64 tiles survive the sampled check.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import opencuda_amd as oc
+from opencuda_amd import surrogate as sg
+
+pytestmark = pytest.mark.gpu
+
+LAYERS = [(2, 32), (32, 32), (32, 32), (32, 32), (32, 32), (32, 1)]
+
+
+def _unpack(p):
+    out, o = [], 0
+    for cin, cout in LAYERS:
+        W = p[o:o + 9 * cin * cout].reshape(9, cin, cout)
+        o += 9 * cin * cout
+        b = p[o:o + cout]
+        o += cout
+        out.append((W, b))
+    assert o == p.size
+    return out
+
+
+def _reference(params, x_u8, y_u8, sigma, margin=6):
+    """float64 CPU network with the GPU's BF16 rounding points."""
+    import torch
+    import torch.nn.functional as F
+
+    def bf(t):
+        return t.to(torch.bfloat16).to(torch.float64)
+
+    class RoundSTE(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, t):
+            return bf(t)
+
+        @staticmethod
+        def backward(ctx, g):
+            return g
+
+    layers = _unpack(params.astype(np.float64))
+    ws, bs = [], []
+    for (W, b), (cin, cout) in zip(layers, LAYERS):
+        w = torch.tensor(W, dtype=torch.float64).permute(2, 1, 0).reshape(cout, cin, 3, 3)
+        ws.append(torch.nn.Parameter(w.clone()))
+        bs.append(torch.nn.Parameter(torch.tensor(b, dtype=torch.float64)))
+    x = torch.tensor(x_u8, dtype=torch.float64)
+    n, h, w = x.shape
+    inp = torch.stack([x / 255.0, torch.tensor(sigma, dtype=torch.float64)[:, None, None]
+                       .expand(n, h, w) / 100.0], 1)
+    a = bf(inp)
+    for l in range(5):
+        a = RoundSTE.apply(F.relu(F.conv2d(a, RoundSTE.apply(ws[l]), bs[l], padding=1)))
+    out = F.conv2d(a, RoundSTE.apply(ws[5]), bs[5], padding=1)[:, 0]
+    pred = x / 255.0 + out
+    y = torch.tensor(y_u8, dtype=torch.float64) / 255.0
+    d = (pred - y)[:, margin:h - margin, margin:w - margin]
+    loss = (d * d).mean()
+    loss.backward()
+    grads = []
+    for (cin, cout), wp, bp in zip(LAYERS, ws, bs):
+        grads.append(wp.grad.reshape(cout, cin, 9).permute(2, 1, 0).reshape(-1).numpy())
+        grads.append(bp.grad.numpy())
+    return pred.detach().numpy(), np.concatenate(grads), float(loss)
+
+
+def test_param_count(device):
+    assert oc.lib().bf_sur_param_count() == 37889
+
+
+def test_forward_and_gradients_match_cpu_float64(device):
+    import torch
+    rng = np.random.default_rng(0)
+    net = sg.Surrogate(seed=3)
+    p = net.get_params()
+    p = p + rng.normal(0, 0.02, p.size).astype(np.float32)    # non-zero biases too
+    net.set_params(p)
+    n, h, w = 2, 30, 70
+    x = np.stack([oc.synth_image(w, h, 4, 3, 12, 10.0, 11 + i) for i in range(n)])
+    y = np.clip(x.astype(int) + rng.integers(-9, 10, x.shape), 0, 255).astype(np.uint8)
+    sigma = np.array([20.0, 37.5], np.float32)
+    tx, ty = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
+    ts = torch.from_numpy(sigma).cuda()
+    pred, grads, loss = net.forward_backward(tx, ty, ts)
+    torch.cuda.synchronize()
+    pred, grads, loss = pred.cpu().numpy(), grads.cpu().numpy(), float(loss.item())
+    rpred, rgrads, rloss = _reference(p, x, y, sigma)
+    assert np.abs(pred - rpred).max() < 2e-3
+    assert abs(loss - rloss) / rloss < 1e-2
+    o = 0
+    for l, (cin, cout) in enumerate(LAYERS):
+        for part, size in (("W", 9 * cin * cout), ("b", cout)):
+            g, r = grads[o:o + size], rgrads[o:o + size]
+            o += size
+            rel = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-30)
+            cos = float(np.dot(g, r) / max(np.linalg.norm(g) * np.linalg.norm(r), 1e-30))
+            assert rel < 0.05 and cos > 0.998, f"layer {l + 1} {part}: rel {rel:.4f} cos {cos:.5f}"
+
+
+def _small_data(n=48, size=128, seed=5):
+    return sg.Dataset(n_images=n, size=size, seed=seed)
+
+
+def test_training_lowers_loss_and_is_deterministic(device):
+    import torch
+    data = _small_data()
+    hist_a = sg.train(sg.Surrogate(seed=9), data, steps=120, batch=16, crop=96, log_every=10)
+    assert hist_a[-1][1] < 0.25 * hist_a[0][1], hist_a
+    a, b = sg.Surrogate(seed=4), sg.Surrogate(seed=4)
+    sg.train(a, data, steps=5, batch=8, crop=64, seed=3)
+    sg.train(b, data, steps=5, batch=8, crop=64, seed=3)
+    torch.cuda.synchronize()
+    assert np.array_equal(a.get_params(), b.get_params())
+
+
+def _image_and_exact(size=320, seed=77, sr=30.0, mode=oc.MODE_STD):
+    import torch
+    img = oc.synth_image(size, size, 12, 16, 160, 12.0, seed)
+    lut = oc.Lut.from_sigmas(10.0, sr)
+    tin = torch.from_numpy(img).cuda()
+    tex = torch.zeros_like(tin)
+    oc.exact(oc.view(tin), oc.view(tex), lut, 15, mode)
+    torch.cuda.synchronize()
+    return img, tin, tex, lut
+
+
+def test_fallback_everywhere_is_exact(device):
+    import torch
+    img, tin, tex, lut = _image_and_exact()
+    net = sg.Surrogate(seed=1)
+    out = torch.zeros_like(tin)
+    total, fb = net.filter(oc.view(tin), oc.view(out), lut, 15, oc.MODE_STD, 30.0, tol_lsb=-1)
+    assert total == fb == 25
+    assert torch.equal(out, tex)
+
+
+def test_no_fallback_is_pure_surrogate(device):
+    import torch
+    img, tin, tex, lut = _image_and_exact()
+    net = sg.Surrogate(seed=1)
+    a, b = torch.zeros_like(tin), torch.zeros_like(tin)
+    total, fb = net.filter(oc.view(tin), oc.view(a), lut, 15, oc.MODE_STD, 30.0, tol_lsb=255)
+    assert fb == 0
+    net.infer(oc.view(tin), oc.view(b), 30.0)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def _within(a, b, tol=2):
+    return float((np.abs(a.astype(int) - b.astype(int)) <= tol).mean())
+
+
+def test_trained_network_against_the_method_without_it(device):
+    """Trained net vs identity (no network: output = input) and vs the
+    untrained network, on held-out images (r=15, STD target)."""
+    import torch
+    net = sg.Surrogate(seed=2)
+    if os.path.exists(sg.DEFAULT_WEIGHTS):
+        net.load(sg.DEFAULT_WEIGHTS)
+    else:
+        sg.train(net, _small_data(n=96, size=160, seed=8), steps=300, batch=16, crop=128)
+    untrained = sg.Surrogate(seed=2)
+    img, tin, tex, lut = _image_and_exact(size=512, seed=1234, sr=25.0)
+    ex = tex.cpu().numpy()
+    outs = {}
+    for name, model in (("trained", net), ("untrained", untrained)):
+        o = torch.zeros_like(tin)
+        model.infer(oc.view(tin), oc.view(o), 25.0)
+        torch.cuda.synchronize()
+        outs[name] = o.cpu().numpy()
+    outs["identity"] = img
+    inner = (slice(16, -16), slice(16, -16))
+    score = {k: _within(v[inner], ex[inner]) for k, v in outs.items()}
+    mae = {k: float(np.abs(v[inner].astype(int) - ex[inner]).mean()) for k, v in outs.items()}
+    print("within 2 LSB:", score, "MAE:", mae)
+    assert score["trained"] > score["identity"] + 0.2
+    assert score["trained"] > score["untrained"] + 0.2
+    assert mae["trained"] < 0.5 * mae["identity"]
