@@ -17,6 +17,11 @@ MI355X FP32 vector nameplate (157.3 TFLOP/s, which assumes packed FMA).
 cpu_baseline: the oracle (host transcription, OpenMP) on a band of rows of
 the same image, timed on this host's cores; the band is also compared
 bit-exactly with the GPU output.
+
+surrogate: the BF16 MFMA surrogate with the sampled exact check and exact
+fallback of failing 64x64 tiles (tolerance 2 LSB), against the exact filter
+in STD mode -- the mode the network is trained on -- with error statistics
+over all pixels.
 """
 import argparse
 import json
@@ -85,6 +90,69 @@ def fp64_error(img, it, gpu_out, rows):
     return {"rows": list(rows), "mean": round(float(err.mean()), 4),
             "mean_abs": round(float(np.abs(err).mean()), 4),
             "max_abs": round(float(np.abs(err).max()), 4)}
+
+
+def surrogate_leg(args, it, img, tin, lut, std_out, std_ms, ref_ms, stream):
+    """BF16 surrogate + sampled exact check + exact fallback (tolerance 2
+    LSB, 64x64 tiles, 1/64 samples) against the exact filter in STD mode."""
+    import numpy as np
+    import torch
+
+    import opencuda_amd as oc
+    from opencuda_amd import surrogate as sg
+    if not os.path.exists(sg.DEFAULT_WEIGHTS):
+        return None
+    net = sg.Surrogate(seed=1)
+    meta = net.load(sg.DEFAULT_WEIGHTS)
+    px = it.pixels
+    sout = torch.zeros_like(tin)
+    vin, vs = oc.view(tin), oc.view(sout)
+    total, fb = net.filter(vin, vs, lut, it.radius, oc.MODE_STD, it.sigma_range, 2, stream=stream)
+    steps = max(3, args.steps // 2)
+    for _ in range(2):
+        net.filter(vin, vs, lut, it.radius, oc.MODE_STD, it.sigma_range, 2, stream=stream,
+                   counts=False)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        net.filter(vin, vs, lut, it.radius, oc.MODE_STD, it.sigma_range, 2, stream=stream,
+                   counts=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    sf_ms = e0.elapsed_time(e1) / steps
+    got = sout.cpu().numpy()
+    # pure surrogate (no check, no fallback)
+    pout = torch.zeros_like(tin)
+    net.infer(vin, oc.view(pout), it.sigma_range, stream=stream)
+    e0.record(stream)
+    for _ in range(steps):
+        net.infer(vin, oc.view(pout), it.sigma_range, stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    inf_ms = e0.elapsed_time(e1) / steps
+    pure = pout.cpu().numpy()
+    d = np.abs(got.astype(np.int16) - std_out.astype(np.int16))
+    dp = np.abs(pure.astype(np.int16) - std_out.astype(np.int16))
+    inf_flop = 75456.0 * px
+    log(f"surrogate+fallback: {sf_ms:.3f} ms ({px / sf_ms / 1e3:.0f} Mpix/s), "
+        f"{fb}/{total} tiles fell back; pure inference {inf_ms:.3f} ms")
+    return {
+        "mpix_s": round(px / (sf_ms / 1e3) / 1e6, 2),
+        "ms": round(sf_ms, 4),
+        "vs_exact_std": round(std_ms / sf_ms, 4),
+        "vs_exact_ref": round(ref_ms / sf_ms, 4),
+        "tiles_total": total, "tiles_fallback": fb,
+        "fallback_fraction": round(fb / total, 4),
+        "err_vs_exact_std": {"max": int(d.max()), "mean": round(float(d.mean()), 4),
+                             "within_2lsb": round(float((d <= 2).mean()), 6)},
+        "pure_inference": {"mpix_s": round(px / (inf_ms / 1e3) / 1e6, 2), "ms": round(inf_ms, 4),
+                           "tflops_bf16": round(inf_flop / (inf_ms / 1e3) / 1e12, 2),
+                           "err_max": int(dp.max()), "err_mean": round(float(dp.mean()), 4),
+                           "within_2lsb": round(float((dp <= 2).mean()), 6)},
+        "target": "exact STD mode (own window, 961 taps)",
+        "weights": os.path.relpath(sg.DEFAULT_WEIGHTS, ROOT),
+        "trained_steps": int(meta.get("steps", 0)),
+    }
 
 
 def main() -> int:
@@ -171,6 +239,9 @@ def main() -> int:
     torch.cuda.synchronize()
     std_ms = e0.elapsed_time(e1) / nstd
 
+    std_out = tout.cpu().numpy()     # exact STD result (the surrogate's target)
+    sur = surrogate_leg(args, it, img, tin, lut, std_out, std_ms, ms_per_step, stream)
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "exact_c3_traffic.json")
     if os.path.exists(tpath):
@@ -213,6 +284,7 @@ def main() -> int:
             "hbm_gbs_algorithmic": round(2 * px / (kern_ms / 1e3) / 1e9, 1),
         },
     }
+    result["surrogate"] = sur
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         base, match, rows = cpu_baseline(img, it, gpu_out, args.cpu_seconds)
