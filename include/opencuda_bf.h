@@ -143,6 +143,27 @@ int bf_batch_run_graph(bf_batch batch, void *stream);
 /* Pixels the batch filters (sum of min(in,out) areas). */
 int64_t bf_batch_pixels(bf_batch batch);
 
+/* Autotuner (no reference counterpart).  bf_tune_radius times every tile
+ * configuration for (radius, mode) on the current device -- a synthetic
+ * width x height image, sigma_space = radius/1.5, sigma_range = 30, `reps`
+ * timed launches each after one warm-up -- and installs the fastest in a
+ * process-wide table.  bf_exact and bf_batch_create (config < 0) use that
+ * table: a single image takes the tuned configuration of its radius; a
+ * batch whose radii are all tuned takes the configuration with the least
+ * estimated total time (sum over jobs of pixels x measured cost).
+ *   best_config: receives the winner; ms_per_mpix (bf_tune_count() floats,
+ *   may be NULL): measured cost of each configuration, < 0 if it could not
+ *   run.  Synchronous. */
+int bf_tune_radius(int32_t radius, int32_t mode, int32_t width, int32_t height,
+                   int32_t reps, int32_t *best_config, float *ms_per_mpix);
+/* Install / read a table entry (e.g. from a saved tuning run):
+ * ms_per_mpix as above.  bf_tune_get returns the tuned configuration, or
+ * -1 if (radius, mode) has not been tuned. */
+int bf_tune_set(int32_t radius, int32_t mode, const float *ms_per_mpix);
+int bf_tune_get(int32_t radius, int32_t mode, float *ms_per_mpix);
+/* Forget every tuned entry (back to the built-in defaults). */
+int bf_tune_reset(void);
+
 /* bf_exact with an explicit tile configuration (the autotuner's probe). */
 int bf_exact_config(const bf_image_view *in, const bf_image_view *out,
                     bf_lut lut, int32_t radius, int32_t mode, int32_t config,

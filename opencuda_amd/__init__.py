@@ -75,6 +75,10 @@ def lib() -> ctypes.CDLL:
         "bf_exact_inplace": ([VP, P, I32, I32, I32, P, P], ctypes.c_int),
         "bf_tune_count": ([], ctypes.c_int),
         "bf_tune_describe": ([I32, P, P, P, P], ctypes.c_int),
+        "bf_tune_radius": ([I32, I32, I32, I32, I32, P, P], ctypes.c_int),
+        "bf_tune_set": ([I32, I32, P], ctypes.c_int),
+        "bf_tune_get": ([I32, I32, P], ctypes.c_int),
+        "bf_tune_reset": ([], ctypes.c_int),
         "bf_batch_create": ([P, I32, I32, ctypes.POINTER(P)], ctypes.c_int),
         "bf_batch_destroy": ([P], ctypes.c_int),
         "bf_batch_run": ([P, P], ctypes.c_int),
@@ -265,4 +269,4 @@ def synth_image(width: int, height: int, nshapes: int, smin: int, smax: int,
     return out
 
 
-from . import workloads  # noqa: E402  (configs of the benchmark suite)
+from . import autotune, workloads  # noqa: E402,F401

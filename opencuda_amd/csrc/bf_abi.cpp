@@ -49,6 +49,34 @@ struct LutFull : Lut {
 
 Lut *lut_from_handle(bf_lut h) { return reinterpret_cast<Lut *>(h); }
 
+// Autotuner results: measured ms per Mpix of each tile configuration for
+// (mode, radius); an entry is "tuned" when at least one cost is >= 0.
+struct TuneTable {
+    std::mutex mu;
+    bool have[2][BF_MAX_RADIUS + 1] = {};
+    float cost[2][BF_MAX_RADIUS + 1][16] = {};
+};
+
+static TuneTable &tune_table()
+{
+    static TuneTable *t = new TuneTable();
+    return *t;
+}
+
+// Tuned configuration for (radius, mode), or -1.
+static int tuned_config(int radius, int mode)
+{
+    if (radius < 1 || radius > BF_MAX_RADIUS || (mode != 0 && mode != 1)) return -1;
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    if (!t.have[mode][radius]) return -1;
+    int best = -1;
+    for (int c = 0; c < tile_config_count(); c++)
+        if (t.cost[mode][radius][c] >= 0 && (best < 0 || t.cost[mode][radius][c] < t.cost[mode][radius][best]))
+            best = c;
+    return best;
+}
+
 static int ntaps_for(int radius, int mode)
 {
     const int g = (2 * radius + 1) * (2 * radius + 1);
@@ -345,9 +373,103 @@ int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int3
         return kNullPointer;
     }
     const int rows = fused_rows(lut_from_handle(lut), radius, mode);
-    const int config = rows < 0 ? default_config(1, mode)
-                                : fit_config(default_config(radius, mode), radius, rows);
+    const int tuned = tuned_config(radius, mode);
+    const int want = tuned >= 0 ? tuned : default_config(radius, mode);
+    const int config = rows < 0 ? default_config(1, mode) : fit_config(want, radius, rows);
     return bf_exact_config(in, out, lut, radius, mode, config, stream);
+}
+
+int bf_tune_radius(int32_t radius, int32_t mode, int32_t width, int32_t height, int32_t reps,
+                   int32_t *best_config, float *ms_per_mpix)
+{
+    if (radius < 1 || radius > BF_MAX_RADIUS || (mode != 0 && mode != 1) || width < 1 ||
+        height < 1 || reps < 1) {
+        set_error("bad tuning parameters");
+        return kInvalidData;
+    }
+    const int nc = tile_config_count();
+    std::vector<unsigned char> host((size_t)width * height);
+    bf_synth_image(host.data(), width, height, std::max(1, width * height / 20000), 8,
+                   std::max(9, std::min(width, height) / 4), 12.0, 0x7e57u + radius);
+    std::vector<int32_t> off(41 * 41 * 2);
+    std::vector<float> g(41 * 41), E(256);
+    bf_gauss_template(std::max(0.5f, radius / 1.5f), 41, off.data(), g.data());
+    bf_euclid_template(30.0f, 256, E.data());
+    bf_lut lut = nullptr;
+    int rc = bf_lut_create(off.data(), g.data(), 41 * 41, E.data(), &lut);
+    if (rc) return rc;
+    unsigned char *din = nullptr, *dout = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<float> cost(nc, -1.0f);
+    hipError_t e = hipMalloc(&din, host.size());
+    if (e == hipSuccess) e = hipMalloc(&dout, host.size());
+    if (e == hipSuccess) e = hipMemcpy(din, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) rc = hip_fail(e, "bf_tune_radius");
+    bf_image_view vin{ din, width, height, width }, vout{ dout, width, height, width };
+    for (int c = 0; c < nc && rc == kOk; c++) {
+        if (bf_exact_config(&vin, &vout, lut, radius, mode, c, nullptr) != kOk) continue;
+        hipEventRecord(e0, nullptr);
+        int r2 = kOk;
+        for (int k = 0; k < reps && r2 == kOk; k++)
+            r2 = bf_exact_config(&vin, &vout, lut, radius, mode, c, nullptr);
+        hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = kHipError;
+            break;
+        }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (r2 == kOk) cost[c] = ms / reps / ((float)width * height / 1e6f);
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    hipFree(din);
+    hipFree(dout);
+    bf_lut_destroy(lut);
+    if (rc) return rc;
+    int rc2 = bf_tune_set(radius, mode, cost.data());
+    if (rc2) return rc2;
+    if (best_config) *best_config = tuned_config(radius, mode);
+    if (ms_per_mpix) std::copy(cost.begin(), cost.end(), ms_per_mpix);
+    return kOk;
+}
+
+int bf_tune_set(int32_t radius, int32_t mode, const float *ms_per_mpix)
+{
+    if (!ms_per_mpix) return kNullPointer;
+    if (radius < 1 || radius > BF_MAX_RADIUS || (mode != 0 && mode != 1)) return kInvalidData;
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    bool any = false;
+    for (int c = 0; c < tile_config_count(); c++) {
+        t.cost[mode][radius][c] = ms_per_mpix[c];
+        any = any || ms_per_mpix[c] >= 0;
+    }
+    t.have[mode][radius] = any;
+    return kOk;
+}
+
+int bf_tune_get(int32_t radius, int32_t mode, float *ms_per_mpix)
+{
+    if (radius < 1 || radius > BF_MAX_RADIUS || (mode != 0 && mode != 1)) return kInvalidData;
+    const int best = tuned_config(radius, mode);
+    if (best >= 0 && ms_per_mpix) {
+        TuneTable &t = tune_table();
+        std::lock_guard<std::mutex> lock(t.mu);
+        for (int c = 0; c < tile_config_count(); c++) ms_per_mpix[c] = t.cost[mode][radius][c];
+    }
+    return best;
+}
+
+int bf_tune_reset(void)
+{
+    TuneTable &t = tune_table();
+    std::lock_guard<std::mutex> lock(t.mu);
+    for (auto &m : t.have)
+        for (auto &r : m) r = false;
+    return kOk;
 }
 
 int bf_exact_inplace(const bf_image_view *img, bf_lut lut, int32_t radius, int32_t mode,
@@ -429,8 +551,37 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
         return kInvalidData;
     }
     const int halo = halo_class(rmax);
-    // Validate and size every job with a provisional config.
+    // Validate and size every job with a provisional config: the one with
+    // the least estimated time when every (radius, mode) is tuned, else the
+    // built-in default for the largest radius.
     int cfg = config >= 0 ? config : default_config(rmax, jobs[0].mode);
+    if (config < 0) {
+        TuneTable &t = tune_table();
+        std::lock_guard<std::mutex> lock(t.mu);
+        bool all = true;
+        for (int i = 0; i < njobs && all; i++) {
+            const int r = jobs[i].radius, m = jobs[i].mode;
+            all = r >= 1 && r <= BF_MAX_RADIUS && (m == 0 || m == 1) && t.have[m][r];
+        }
+        if (all) {
+            double best = -1.0;
+            for (int c = 0; c < tile_config_count(); c++) {
+                double est = 0.0;
+                bool ok = true;
+                for (int i = 0; i < njobs && ok; i++) {
+                    const float k = t.cost[jobs[i].mode][jobs[i].radius][c];
+                    if (k < 0) ok = false;
+                    const double px = (double)std::min(jobs[i].in.width, jobs[i].out.width) *
+                                      std::min(jobs[i].in.height, jobs[i].out.height);
+                    est += px * k;
+                }
+                if (ok && (best < 0 || est < best)) {
+                    best = est;
+                    cfg = c;
+                }
+            }
+        }
+    }
     if (cfg >= tile_config_count()) {
         set_error("bad tile config");
         return kInvalidData;

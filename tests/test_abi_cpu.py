@@ -133,3 +133,19 @@ def test_public_headers_compile_with_plain_gxx(tmp_path):
     r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_tune_table_without_gpu():
+    from opencuda_amd import autotune
+    n = len(oc.tune_configs())
+    try:
+        cost = [0.5 + 0.1 * c for c in range(n)]
+        cost[3] = 0.1
+        autotune.install(7, oc.MODE_STD, cost)
+        best, got = autotune.get(7, oc.MODE_STD)
+        assert best == 3 and np.allclose(got, cost)
+        assert autotune.get(7, oc.MODE_REF)[0] == -1
+        autotune.install(7, oc.MODE_STD, [-1.0] * n)      # nothing usable -> untuned
+        assert autotune.get(7, oc.MODE_STD)[0] == -1
+    finally:
+        autotune.reset()

@@ -165,3 +165,18 @@ def test_c3_parity_full_size(device):
                           rows=(4096, 4160))
     err = got[4096:4160].astype(np.float64) - v
     assert np.abs(err).max() < 1.05 and -0.75 < err.mean() < -0.25
+
+
+def test_autotuner_installs_a_working_config(device):
+    from opencuda_amd import autotune
+    try:
+        best, cost = autotune.tune(5, oc.MODE_REF, 512, 256, reps=1)
+        assert 0 <= best < len(oc.tune_configs())
+        assert cost[best] == min(c for c in cost if c >= 0)
+        assert autotune.get(5, oc.MODE_REF)[0] == best
+        img = _img(120, 130, seed=17)
+        got = run_exact(img, 5, 3.3, 22.0, oc.MODE_REF)      # uses the tuned config
+        check_against_oracle(img, 5, 3.3, 22.0, oc.MODE_REF, got=got)
+    finally:
+        autotune.reset()
+    assert autotune.get(5, oc.MODE_REF)[0] == -1
