@@ -226,6 +226,16 @@ int bf_sur_filter(bf_sur sur, const bf_image_view *in, const bf_image_view *out,
                   int32_t tol_lsb, int32_t *tiles_total, int32_t *tiles_fallback,
                   void *stream);
 
+/* bf_sur_filter without host synchronisation, safe to capture into a
+ * hipGraph once the network's workspace has grown to the largest image
+ * (run each size once outside the capture): adds the number of tiles that
+ * fell back to *fallback_accum and the number of tiles to *tiles_accum
+ * (device ints, either may be NULL). */
+int bf_sur_filter_async(bf_sur sur, const bf_image_view *in, const bf_image_view *out,
+                        bf_lut lut, int32_t radius, int32_t mode, float sigma_range,
+                        int32_t tol_lsb, int32_t *fallback_accum, int32_t *tiles_accum,
+                        void *stream);
+
 /* ---------------------------------------------------------------------
  * Misc.
  */

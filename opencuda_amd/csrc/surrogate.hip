@@ -94,11 +94,12 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
     __shared__ float s_first[(KIND == K_FIRST || KIND == K_DG_LAST) ? (TY + 2) * (TX + 2) : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, n = blockIdx.z;
     const int H = a.H, W = a.W;
     const int64_t npix = (int64_t)H * W;
+    const int tiles_x = (W + TX - 1) / TX, tiles_y = (H + TY - 1) / TY;
+    const int64_t ntiles = (int64_t)tiles_x * tiles_y * a.N;
 
-    // ---- B operand (weights) into registers
+    // ---- B operand (weights) into registers, once per persistent workgroup
     bf16x8 bw[T::KS][T::NT];
 #pragma unroll
     for (int s = 0; s < T::KS; s++)
@@ -106,6 +107,12 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
         for (int nt = 0; nt < T::NT; nt++)
             bw[s][nt] = *reinterpret_cast<const bf16x8 *>(
                 a.wpk + ((size_t)(nt * 16 + (lane & 15)) * T::KTOT + s * 32 + 8 * (lane >> 4)));
+
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int n = (int)(tile / ((int64_t)tiles_x * tiles_y));
+    const int trem = (int)(tile - (int64_t)n * tiles_x * tiles_y);
+    const int y0 = (trem / tiles_x) * TY, x0 = (trem % tiles_x) * TX;
+    __syncthreads();   // the previous tile's reads of the staging buffer are done
 
     // ---- stage the input tile (rows y0-1 .. y0+4, cols x0-1 .. x0+64), zero outside
     if (T::STAGE32) {
@@ -194,7 +201,7 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
 
     // ---- epilogue
     const int y = y0 + wave;
-    if (y >= H) return;
+    if (y < H) {
 #pragma unroll
     for (int m = 0; m < 4; m++) {
 #pragma unroll
@@ -228,6 +235,8 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
             }
         }
     }
+    }
+    }   // tile loop
 }
 
 // ------------------------------------------------------------ weight grads
@@ -489,6 +498,8 @@ struct CheckArgs {
     int32_t *flags;              // tiles_x * tiles_y
     int32_t *list;               // fallback tile ids
     int32_t *count;
+    int32_t *accum;              // optional running total of fallback tiles
+    int32_t *tiles_accum;        // optional running total of tiles
     uint32_t salt;
 };
 
@@ -543,7 +554,9 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
         a.flags[tile] = 1;
         const int slot = atomicAdd(a.count, 1);
         a.list[slot] = tile;
+        if (a.accum) atomicAdd(a.accum, 1);
     }
+    if (threadIdx.x == 0 && a.tiles_accum) atomicAdd(a.tiles_accum, 1);
 }
 
 __global__ void zero_ints(int32_t *p, int n)
@@ -614,8 +627,16 @@ int pack(Net *net, hipStream_t s)
 template <int KIND>
 int launch_conv(const ConvArgs &a, hipStream_t s)
 {
-    dim3 grid((a.W + TX - 1) / TX, (a.H + TY - 1) / TY, a.N);
-    hipLaunchKernelGGL(conv_kernel<KIND>, grid, dim3(256), 0, s, a);
+    // Persistent: about 4 workgroups per CU walk the 64 x 4 tiles.
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        BF_HIP(hipGetDevice(&dev));
+        BF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const int64_t ntiles = (int64_t)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.N;
+    const int grid = (int)std::min<int64_t>(ntiles, (int64_t)cus * 4);
+    hipLaunchKernelGGL(conv_kernel<KIND>, dim3(grid), dim3(256), 0, s, a);
     BF_HIP(hipGetLastError());
     return kOk;
 }
@@ -932,13 +953,10 @@ int bf_sur_infer(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, 
     return infer_view(net_of(hnd), in, out, sigma_range, (hipStream_t)stream, &W, &H);
 }
 
-int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, bf_lut lut,
-                  int32_t radius, int32_t mode, float sigma_range, int32_t tol_lsb,
-                  int32_t *tiles_total, int32_t *tiles_fallback, void *stream)
+static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *out, bf_lut lut,
+                      int radius, int mode, float sigma_range, int tol_lsb, int32_t *accum,
+                      int32_t *tiles_accum, hipStream_t s)
 {
-    if (!hnd || !lut) return kNullPointer;
-    Net *net = net_of(hnd);
-    hipStream_t s = (hipStream_t)stream;
     // exact job for the check and the fallback (64 x 64 tiles = config 0)
     DevJob job;
     int rc = make_job(in, out, lut_from_handle(lut), radius, mode, 0, halo_class(radius), 0, &job,
@@ -951,6 +969,7 @@ int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out,
         hipFree(net->flags);
         hipFree(net->list);
         net->flags = net->list = nullptr;
+        net->fb_tiles = 0;
         BF_HIP(hipMalloc(&net->flags, nt * 4));
         BF_HIP(hipMalloc(&net->list, nt * 4));
         net->fb_tiles = nt;
@@ -968,12 +987,26 @@ int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out,
     ca.flags = net->flags;
     ca.list = net->list;
     ca.count = net->count;
+    ca.accum = accum;
+    ca.tiles_accum = tiles_accum;
     ca.salt = 0x5bd1e995u;
     hipLaunchKernelGGL(check_kernel, dim3(nt), dim3(64), 0, s, ca);
     BF_HIP(hipGetLastError());
     // exact fallback over the listed tiles
-    if ((rc = exact_tiles(job, net->list, nt, net->count, 0, halo_class(radius), s))) return rc;
-    if (tiles_total) *tiles_total = nt;
+    return exact_tiles(job, net->list, nt, net->count, 0, halo_class(radius), s);
+}
+
+int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, bf_lut lut,
+                  int32_t radius, int32_t mode, float sigma_range, int32_t tol_lsb,
+                  int32_t *tiles_total, int32_t *tiles_fallback, void *stream)
+{
+    if (!hnd || !lut) return kNullPointer;
+    Net *net = net_of(hnd);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = sur_filter(net, in, out, lut, radius, mode, sigma_range, tol_lsb, nullptr, nullptr, s);
+    if (rc) return rc;
+    const int W = std::min(in->width, out->width), H = std::min(in->height, out->height);
+    if (tiles_total) *tiles_total = ((W + 63) / 64) * ((H + 63) / 64);
     if (tiles_fallback) {
         int32_t c = 0;
         BF_HIP(hipMemcpyAsync(&c, net->count, 4, hipMemcpyDeviceToHost, s));
@@ -981,6 +1014,15 @@ int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out,
         *tiles_fallback = c;
     }
     return kOk;
+}
+
+int bf_sur_filter_async(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, bf_lut lut,
+                        int32_t radius, int32_t mode, float sigma_range, int32_t tol_lsb,
+                        int32_t *fallback_accum, int32_t *tiles_accum, void *stream)
+{
+    if (!hnd || !lut) return kNullPointer;
+    return sur_filter(net_of(hnd), in, out, lut, radius, mode, sigma_range, tol_lsb,
+                      fallback_accum, tiles_accum, (hipStream_t)stream);
 }
 
 }  // extern "C"

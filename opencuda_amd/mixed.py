@@ -1,0 +1,167 @@
+"""Mixed workload: exact filter for small radii and the learned surrogate
+with exact fallback for large radii, all in one captured graph.
+
+Images with r >= 9 (here r = 15, the radius the shipped network is trained
+for) run bf_sur_filter_async (surrogate + sampled exact check + exact
+fallback of failing 64x64 tiles); the others run in one persistent
+bf_batch launch.  The whole sequence is captured once into a graph and
+replayed.  Every image uses the same filter mode (STD by default: the mode
+the network approximates), so "exact" means the same thing for both groups.
+
+    python -m opencuda_amd.mixed [--count 512] [--steps 3]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import Batch, Job, Lut, MODE_REF, MODE_STD, exact, view
+from . import workloads as wl
+from .surrogate import DEFAULT_WEIGHTS, Surrogate
+
+
+class Mixed:
+    def __init__(self, items, net: Surrogate, mode: int = MODE_STD, sur_min_radius: int = 9,
+                 tol: int = 2, log=None):
+        import torch
+        self.items = items
+        self.mode = mode
+        self.tol = tol
+        self.net = net
+        self.luts = {}
+        self.tin, self.tout = [], []
+        self.sur_idx, self.ex_idx = [], []
+        t0 = time.time()
+        for i, it in enumerate(items):
+            img = it.image()
+            x1, y1, x2, y2 = it.roi
+            self.tin.append(torch.from_numpy(img).cuda())
+            self.tout.append(torch.zeros((y2 - y1, x2 - x1), dtype=torch.uint8, device="cuda"))
+            key = (it.sigma_space, it.sigma_range)
+            if key not in self.luts:
+                self.luts[key] = Lut.from_sigmas(*key)
+            (self.sur_idx if it.radius >= sur_min_radius else self.ex_idx).append(i)
+            if log and (i + 1) % 128 == 0:
+                log(f"{i + 1}/{len(items)} images uploaded ({time.time() - t0:.1f} s)")
+        jobs = [Job(view(self.tin[i], items[i].roi), view(self.tout[i]),
+                    self.luts[(items[i].sigma_space, items[i].sigma_range)].handle,
+                    items[i].radius, mode) for i in self.ex_idx]
+        self.batch = Batch(jobs) if jobs else None
+        self.fallback = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.tiles = torch.zeros(1, dtype=torch.int32, device="cuda")
+        self.graph = None
+        self.pixels = sum(it.pixels for it in items)
+        self.sur_pixels = sum(items[i].pixels for i in self.sur_idx)
+
+    def _sur_order(self):
+        # largest first, so the network's workspace reaches its final size
+        # on the first image of the eager warm-up
+        return sorted(self.sur_idx, key=lambda i: -self.items[i].pixels)
+
+    def run(self, stream=None):
+        """The sequence the graph holds: batch launch, then each surrogate image."""
+        if self.batch is not None:
+            self.batch.run(stream)
+        for i in self._sur_order():
+            it = self.items[i]
+            self.net.filter_async(view(self.tin[i], it.roi), view(self.tout[i]),
+                                  self.luts[(it.sigma_space, it.sigma_range)], it.radius,
+                                  self.mode, it.sigma_range, self.tol, self.fallback, self.tiles,
+                                  stream)
+
+    def capture(self):
+        import torch
+        self.run()                      # eager warm-up: grows every workspace
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.run()
+        torch.cuda.synchronize()
+
+    def replay(self):
+        self.graph.replay()
+
+    def reset_counts(self):
+        self.fallback.zero_()
+        self.tiles.zero_()
+
+    def counts(self):
+        import torch
+        torch.cuda.synchronize()
+        return int(self.tiles.item()), int(self.fallback.item())
+
+    def exact_reference(self, i):
+        """The exact filter of image i (same mode), computed on the GPU."""
+        import torch
+        it = self.items[i]
+        ref = torch.zeros_like(self.tout[i])
+        exact(view(self.tin[i], it.roi), view(ref), self.luts[(it.sigma_space, it.sigma_range)],
+              it.radius, self.mode)
+        torch.cuda.synchronize()
+        return ref
+
+
+def measure(count=512, steps=3, mode=MODE_STD, log=None):
+    import torch
+    net = Surrogate(seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    mx = Mixed(wl.c4(count), net, mode=mode, log=log)
+    if log:
+        log(f"{len(mx.ex_idx)} exact images, {len(mx.sur_idx)} surrogate images, "
+            f"{mx.pixels / 1e6:.1f} Mpx; capturing")
+    mx.capture()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    mx.replay()
+    torch.cuda.synchronize()
+    mx.reset_counts()
+    e0.record()
+    for _ in range(steps):
+        mx.replay()
+    e1.record()
+    tiles, fb = mx.counts()
+    ms = e0.elapsed_time(e1) / steps
+    # errors of the surrogate images against the exact filter (all pixels)
+    errs, within = [], []
+    for i in mx.sur_idx:
+        got = mx.tout[i].cpu().numpy().astype(np.int16)
+        ref = mx.exact_reference(i).cpu().numpy().astype(np.int16)
+        d = np.abs(got - ref)
+        errs.append((int(d.max()), float(d.sum()), d.size))
+        within.append(int((d <= 2).sum()))
+    npx = sum(e[2] for e in errs)
+    out = {
+        "workload": f"c4: {count} images 128..4096 per side, r in {{3,7,15}}, "
+                    f"{'STD' if mode == MODE_STD else 'REF'} mode",
+        "images": count, "exact_images": len(mx.ex_idx), "surrogate_images": len(mx.sur_idx),
+        "pixels": mx.pixels, "ms_per_graph": round(ms, 3),
+        "mpix_s": round(mx.pixels / (ms / 1e3) / 1e6, 1),
+        "tiles_checked": tiles // steps, "tiles_fallback": fb // steps,
+        "fallback_fraction": round(fb / max(tiles, 1), 4),
+        "sur_err_vs_exact": {"max": max((e[0] for e in errs), default=0),
+                             "mean": round(sum(e[1] for e in errs) / max(npx, 1), 4),
+                             "within_2lsb": round(sum(within) / max(npx, 1), 6)},
+    }
+    return out, mx
+
+
+def _main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--count", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--mode", choices=["std", "ref"], default="std")
+    args = ap.parse_args()
+
+    def log(m):
+        print(f"[mixed] {m}", file=sys.stderr, flush=True)
+
+    res, _ = measure(args.count, args.steps, MODE_STD if args.mode == "std" else MODE_REF, log=log)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main())

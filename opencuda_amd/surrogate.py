@@ -101,6 +101,18 @@ class Surrogate:
                "bf_sur_filter")
         return (tt.value, tf.value) if counts else None
 
+    def filter_async(self, vin: View, vout: View, lut: Lut, radius: int, mode: int,
+                     sigma_range: float, tol_lsb: int = 2, fallback_accum=None,
+                     tiles_accum=None, stream=None) -> None:
+        """bf_sur_filter_async: no host synchronisation (graph-capturable);
+        fallback/tile counts are added to the given device int32 tensors."""
+        fa = ctypes.c_void_p(fallback_accum.data_ptr()) if fallback_accum is not None else None
+        ta = ctypes.c_void_p(tiles_accum.data_ptr()) if tiles_accum is not None else None
+        _check(lib().bf_sur_filter_async(self.handle, ctypes.byref(vin), ctypes.byref(vout),
+                                         lut.handle, int(radius), int(mode), float(sigma_range),
+                                         int(tol_lsb), fa, ta, _stream(stream)),
+               "bf_sur_filter_async")
+
     def close(self):
         if self.handle:
             lib().bf_sur_destroy(self.handle)
