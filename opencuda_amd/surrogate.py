@@ -176,8 +176,11 @@ class Dataset:
 
 
 def train(net: Surrogate, data: Dataset, steps: int = 1500, batch: int = 64, crop: int = 256,
-          lr: float = 1e-3, seed: int = 7, log=None, log_every: int = 100):
-    """Adam on random crops; returns the list of (step, loss)."""
+          lr: float = 1e-3, seed: int = 7, log=None, log_every: int = 100, cosine: bool = False):
+    """Adam on random crops (learning rate `lr`, or `lr` decayed to 0 along
+    a half cosine when `cosine`); returns the list of (step, loss)."""
+    import math
+
     import torch
     rng = np.random.default_rng(seed)
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
@@ -185,7 +188,8 @@ def train(net: Surrogate, data: Dataset, steps: int = 1500, batch: int = 64, cro
     t0 = time.time()
     for s in range(1, steps + 1):
         x, y, sg = data.batch(rng, batch, crop)
-        net.train_step(x, y, sg, lr=lr, loss=loss)
+        lr_s = lr * 0.5 * (1.0 + math.cos(math.pi * (s - 1) / steps)) if cosine else lr
+        net.train_step(x, y, sg, lr=lr_s, loss=loss)
         if s % log_every == 0 or s == 1 or s == steps:
             v = float(loss.item())
             hist.append((s, v))
@@ -193,6 +197,30 @@ def train(net: Surrogate, data: Dataset, steps: int = 1500, batch: int = 64, cro
                 log(f"step {s} loss {v:.3e} rmse {255 * v ** 0.5:.3f} LSB ({time.time() - t0:.1f} s)")
     torch.cuda.synchronize()
     return hist
+
+
+def evaluate(net: Surrogate, img: np.ndarray, sigma_space: float, sigma_range: float,
+             radius: int = 15, mode: int = MODE_STD, tol: int = 2) -> dict:
+    """Accuracy of the pure surrogate against the exact filter on one image:
+    fraction of pixels within `tol`, fraction of 64x64 tiles entirely within
+    `tol`, and the fraction of tiles the sampled check lets through."""
+    import torch
+    lut = Lut.from_sigmas(sigma_space, sigma_range)
+    tin = torch.from_numpy(img).cuda()
+    ex, su, sf = torch.zeros_like(tin), torch.zeros_like(tin), torch.zeros_like(tin)
+    exact(view(tin), view(ex), lut, radius, mode)
+    net.infer(view(tin), view(su), sigma_range)
+    total, fb = net.filter(view(tin), view(sf), lut, radius, mode, sigma_range, tol)
+    torch.cuda.synchronize()
+    d = (su.to(torch.int16) - ex.to(torch.int16)).abs().cpu().numpy()
+    H, W = d.shape
+    th, tw = (H + 63) // 64, (W + 63) // 64
+    pad = np.zeros((th * 64, tw * 64), d.dtype)
+    pad[:H, :W] = d
+    tile_max = pad.reshape(th, 64, tw, 64).max(axis=(1, 3))
+    return {"within": float((d <= tol).mean()), "mean_abs": float(d.mean()), "max": int(d.max()),
+            "tiles_all_within": float((tile_max <= tol).mean()),
+            "tiles_passing_check": 1.0 - fb / total, "tiles": total}
 
 
 def _main() -> int:
@@ -206,6 +234,8 @@ def _main() -> int:
     ap.add_argument("--mode", choices=["std", "ref"], default="std")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--cosine", action="store_true", help="decay lr along a half cosine")
+    ap.add_argument("--eval", action="store_true", help="report accuracy on the c3 image")
     args = ap.parse_args()
 
     def log(m):
@@ -214,11 +244,16 @@ def _main() -> int:
     mode = MODE_STD if args.mode == "std" else MODE_REF
     data = Dataset(args.images, args.size, mode=mode, log=log)
     net = Surrogate(args.seed)
-    hist = train(net, data, args.steps, args.batch, args.crop, args.lr, log=log)
+    hist = train(net, data, args.steps, args.batch, args.crop, args.lr, log=log,
+                 cosine=args.cosine)
     net.save(args.out, history=np.array(hist, np.float64), mode=mode, radius=15,
              sigma_space=10.0, images=args.images, steps=args.steps, batch=args.batch,
-             crop=args.crop, lr=args.lr, seed=args.seed)
+             crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine))
     log(f"saved {args.out}")
+    if args.eval:
+        from . import workloads
+        (it,) = workloads.c3()
+        log(f"c3 accuracy: {evaluate(net, it.image(), it.sigma_space, it.sigma_range, mode=mode)}")
     return 0
 
 
