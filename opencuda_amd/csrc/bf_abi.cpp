@@ -249,6 +249,40 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
         L->rows_upto[c + 1] = (int)rowg.size();
     }
     L->nrows_total = (int)rowg.size();
+    // Canonical ring template (the GAUSS factory's order, one weight per
+    // q = dy^2 + dx^2)?  Then the unrolled kernels apply; they index the
+    // fused table by q in order of first appearance.
+    std::vector<float> rowq;
+    {
+        bool canon = true;
+        std::unordered_map<int, int> qrow;
+        std::vector<std::pair<int, int>> ring;
+        ring.emplace_back(0, 0);
+        for (int i = 1; (int)ring.size() < count; i++)
+            for (int j = -i + 1; j <= i; j++) {
+                ring.emplace_back(i, j);
+                ring.emplace_back(-i, j);
+                ring.emplace_back(j, i);
+                ring.emplace_back(j, -i);
+            }
+        for (int c = 0; c < count && canon; c++) {
+            const int dy = offsets[2 * c], dx = offsets[2 * c + 1];
+            if (dy != ring[c].first || dx != ring[c].second) {
+                canon = false;
+                break;
+            }
+            const int q = dy * dy + dx * dx;
+            auto it = qrow.find(q);
+            if (it == qrow.end()) {
+                qrow.emplace(q, (int)rowq.size());
+                rowq.push_back(weights[c]);
+            } else if (std::memcmp(&rowq[it->second], &weights[c], 4) != 0) {
+                canon = false;
+            }
+        }
+        L->canonical = canon;
+        if (!canon) rowq.clear();
+    }
     // Tap variants for every radius and mode that fits in the template.
     std::vector<uint32_t> packed;
     std::vector<float> gtap;
@@ -278,6 +312,9 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
     if (e == hipSuccess) e = hipMalloc(&L->d_gtap, gtap.size() * 4);
     if (e == hipSuccess) e = hipMalloc(&L->d_rowg, std::max<size_t>(1, rowg.size()) * 4);
     if (e == hipSuccess) e = hipMalloc(&L->d_range, 256 * 4);
+    if (e == hipSuccess && !rowq.empty()) e = hipMalloc(&L->d_rowq, rowq.size() * 4);
+    if (e == hipSuccess && !rowq.empty())
+        e = hipMemcpy(L->d_rowq, rowq.data(), rowq.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(L->d_taps, packed.data(), packed.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(L->d_gtap, gtap.data(), gtap.size() * 4, hipMemcpyHostToDevice);
     if (e == hipSuccess && !rowg.empty())
@@ -288,6 +325,7 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
         hipFree(L->d_gtap);
         hipFree(L->d_rowg);
         hipFree(L->d_range);
+        hipFree(L->d_rowq);
         delete L;
         return hip_fail(e, "bf_lut_create");
     }
@@ -301,6 +339,7 @@ int bf_lut_destroy(bf_lut h)
     auto *L = static_cast<LutFull *>(lut_from_handle(h));
     hipFree(L->d_taps);
     hipFree(L->d_gtap);
+    hipFree(L->d_rowq);
     hipFree(L->d_rowg);
     hipFree(L->d_range);
     delete L;
@@ -371,6 +410,30 @@ int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int3
     if (!lut) {
         set_error("null lut");
         return kNullPointer;
+    }
+    Lut *L = lut_from_handle(lut);
+    int rcfg = -1;
+    if (L->canonical && ring_available(radius, mode, &rcfg) && ring_rows(radius, mode) > 0 &&
+        ntaps_for(radius, mode) <= L->count) {
+        if (!in || !out) {
+            set_error("null argument");
+            return kNullPointer;
+        }
+        if (in->data && out->data && in->width > 0 && out->width > 0 && in->height > 0 &&
+            out->height > 0 && views_overlap(in, out)) {
+            set_error("input and output views overlap");
+            return kInvalidData;
+        }
+        DevJob job;
+        int rc = make_job(in, out, L, radius, mode, rcfg, halo_class(radius), 0, &job, nullptr);
+        if (rc) return rc;
+        const int rr = ring_rows(radius, mode);
+        if (!job.sep && lds_bytes(rcfg, halo_class(radius), rr + 1) <= max_lds_bytes()) {
+            job.rowg = L->d_rowq;
+            job.nrows = rr;
+            job.lut_rows = rr + 1;
+            return launch_ring(job, radius, mode, tiles_of(job, rcfg), (hipStream_t)stream);
+        }
     }
     const int rows = fused_rows(lut_from_handle(lut), radius, mode);
     const int tuned = tuned_config(radius, mode);

@@ -35,6 +35,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <utility>
 
 #include "bf_internal.h"
 
@@ -343,6 +344,165 @@ bf_exact_kernel(KArgs args)
     }
 }
 
+// ------------------------------------------------------------------------
+// Unrolled variant for the canonical ring template (the one the GAUSS
+// factory emits, TemplateFactory.cu:1092-1168).  The tap list of a radius is
+// known at compile time, so every tap's tile offset and fused-table row are
+// immediates: the per-fetch address add disappears (ds_read offsets), and
+// the compiler schedules across taps freely.  The fused table has one row
+// per distinct q = dy^2 + dx^2 in order of first appearance; the host checks
+// that the template really is the canonical one (same offsets, one weight
+// per q) before choosing this kernel.
+
+struct RingTap {
+    int dy, dx, row;
+};
+
+template <int R, int MODE>
+struct RingTaps {
+    static constexpr int T = (2 * R + 1) * (2 * R + 1) + (MODE == kModeRef ? 1 : 0);
+    RingTap t[T];
+    int nrows;
+    constexpr RingTaps() : t{}, nrows(0)
+    {
+        int qs[T] = {};
+        int n = 0;
+        for (int i = 0; i <= R + 1 && n < T; i++) {
+            for (int j = (i == 0 ? 0 : -i + 1); j <= i && n < T; j++) {
+                const int pts[4][2] = { { i, j }, { -i, j }, { j, i }, { j, -i } };
+                for (int k = 0; k < (i == 0 ? 1 : 4) && n < T; k++) {
+                    const int dy = pts[k][0], dx = pts[k][1], q = dy * dy + dx * dx;
+                    int row = -1;
+                    for (int m = 0; m < nrows; m++)
+                        if (qs[m] == q) row = m;
+                    if (row < 0) {
+                        qs[nrows] = q;
+                        row = nrows++;
+                    }
+                    t[n++] = RingTap{ dy, dx, row };
+                }
+            }
+        }
+    }
+};
+
+template <int R, int MODE>
+inline constexpr RingTaps<R, MODE> kRing{};
+
+// Smallest tap offset (in tile elements) of a radius: the unrolled kernels
+// address every fetch from one base placed there, so all ds_read offsets
+// are non-negative immediates and no per-tap base register is needed.
+template <int R, int MODE, int TCOLS>
+constexpr int ring_min_offset()
+{
+    int m = 0;
+    for (int c = 0; c < RingTaps<R, MODE>::T; c++) {
+        const int o = kRing<R, MODE>.t[c].dy * TCOLS + kRing<R, MODE>.t[c].dx;
+        m = o < m ? o : m;
+    }
+    return m;
+}
+
+// Taps scheduled together in the unrolled kernels (STD has 4 fetches per
+// tap, so it keeps fewer taps in flight).
+template <int MODE>
+constexpr int ring_group() { return MODE == kModeRef ? 4 : 2; }
+
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+    (f(std::integral_constant<int, I>{}), ...);
+}
+
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int R, int MODE, int TW, int TGY, int HALO>
+__device__ __forceinline__ void tile_ring(const DevJob &job, int X0, int Y0, int lx, int gy)
+{
+    using G = Geo<TW, TGY, HALO>;
+    constexpr uint32_t LUT_BASE = kTileOff + G::TILE_BYTES;
+    constexpr int ROWB = G::TCOLS * 4;
+    constexpr int T = RingTaps<R, MODE>::T;
+    const uint32_t lane = kTileOff + (uint32_t)(((4 * gy + HALO) * G::TCOLS + lx + HALO) * 4);
+    const uint32_t c0 = lds_u32(lane), c1 = lds_u32(lane + ROWB);
+    const uint32_t c2 = lds_u32(lane + 2 * ROWB), c3 = lds_u32(lane + 3 * ROWB);
+    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+    constexpr int MINB = ring_min_offset<R, MODE, G::TCOLS>() * 4;
+    const uint32_t tbase = lane + (uint32_t)MINB;   // >= 0: the halo covers every tap
+    // Gathers address W[row][d] as (4d + LUT_BASE) + row*1024; rows past 63
+    // go through a second base so every offset fits the 16-bit field.
+    static_for<T>([&](auto ci) {
+        constexpr RingTap tp = kRing<R, MODE>.t[decltype(ci)::value];
+        constexpr int TAPB = (tp.dy * G::TCOLS + tp.dx) * 4 - MINB;   // >= 0
+        constexpr uint32_t WB = LUT_BASE + (tp.row >= 64 ? 65536u : 0u);
+        constexpr uint32_t WOFF = (uint32_t)(tp.row & 63) * 1024u;
+        if constexpr (MODE == kModeRef) {
+            const uint32_t p = lds_u32(tbase + TAPB);
+            const float pf = (float)(p & 0xFFu);
+            const float f0 = lds_f32(__builtin_amdgcn_sad_u8(p, c0, WB) + WOFF);
+            const float f1 = lds_f32(__builtin_amdgcn_sad_u8(p, c1, WB) + WOFF);
+            const float f2 = lds_f32(__builtin_amdgcn_sad_u8(p, c2, WB) + WOFF);
+            const float f3 = lds_f32(__builtin_amdgcn_sad_u8(p, c3, WB) + WOFF);
+            t0 = t0 + f0 * pf; s0 = s0 + f0;
+            t1 = t1 + f1 * pf; s1 = s1 + f1;
+            t2 = t2 + f2 * pf; s2 = s2 + f2;
+            t3 = t3 + f3 * pf; s3 = s3 + f3;
+        } else {
+            const uint32_t p0 = lds_u32(tbase + TAPB), p1 = lds_u32(tbase + TAPB + ROWB);
+            const uint32_t p2 = lds_u32(tbase + TAPB + 2 * ROWB), p3 = lds_u32(tbase + TAPB + 3 * ROWB);
+            const float f0 = lds_f32(__builtin_amdgcn_sad_u8(p0, c0, WB) + WOFF);
+            const float f1 = lds_f32(__builtin_amdgcn_sad_u8(p1, c1, WB) + WOFF);
+            const float f2 = lds_f32(__builtin_amdgcn_sad_u8(p2, c2, WB) + WOFF);
+            const float f3 = lds_f32(__builtin_amdgcn_sad_u8(p3, c3, WB) + WOFF);
+            t0 = t0 + f0 * (float)(p0 & 0xFFu); s0 = s0 + f0;
+            t1 = t1 + f1 * (float)(p1 & 0xFFu); s1 = s1 + f1;
+            t2 = t2 + f2 * (float)(p2 & 0xFFu); s2 = s2 + f2;
+            t3 = t3 + f3 * (float)(p3 & 0xFFu); s3 = s3 + f3;
+        }
+        // Keep the schedule local (a group of taps in flight), as the loop
+        // version does: without a fence the compiler issues every gather of
+        // the window first and spills them.  The empty asm reads and writes
+        // the accumulators and clobbers memory, so neither the adds nor the
+        // LDS reads move across it; it emits no instruction.
+        if constexpr (decltype(ci)::value % ring_group<MODE>() == ring_group<MODE>() - 1)
+            asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2), "+v"(t3), "+v"(s0), "+v"(s1),
+                         "+v"(s2), "+v"(s3)::"memory");
+    });
+    const int x = X0 + lx, y0 = Y0 + 4 * gy;
+    if (x < job.W) {
+        uint8_t *o = job.out + (int64_t)y0 * job.out_pitch + x;
+        if (y0 < job.H) o[0] = to_u8(t0 / s0);
+        if (y0 + 1 < job.H) o[job.out_pitch] = to_u8(t1 / s1);
+        if (y0 + 2 < job.H) o[2 * job.out_pitch] = to_u8(t2 / s2);
+        if (y0 + 3 < job.H) o[3 * job.out_pitch] = to_u8(t3 / s3);
+    }
+}
+
+// One image, static grid-stride over its tiles; job.rowg holds the
+// per-q weights of the canonical template and job.nrows their count.
+template <int R, int MODE, int TW, int TGY, int HALO>
+__global__ void __launch_bounds__(TW * TGY) bf_ring_kernel(DevJob job, int ntiles)
+{
+    using G = Geo<TW, TGY, HALO>;
+    const int tid = threadIdx.x;
+    const int lx = tid % TW, gy = tid / TW;
+    build_lut<G::NT>(job, kTileOff + (uint32_t)G::TILE_BYTES, tid);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int ty = tile / job.tiles_x;
+        const int X0 = (tile - ty * job.tiles_x) * TW;
+        const int Y0 = ty * G::TH;
+        __syncthreads();                      // table built / previous tile consumed
+        load_tile<TW, TGY, HALO>(job, X0, Y0, tid);
+        __syncthreads();
+        tile_ring<R, MODE, TW, TGY, HALO>(job, X0, Y0, lx, gy);
+    }
+}
+
 __global__ void reset_counter_kernel(int32_t *counter) { *counter = 0; }
 
 // ---------------------------------------------------------------- configs
@@ -467,6 +627,81 @@ int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_max,
     case 0: return launch_halo<64, 16>(halo, a, job.lut_rows, s);
     case 1: return launch_halo<64, 8>(halo, a, job.lut_rows, s);
     default: set_error("fallback needs a 64-wide tile config"); return kInvalidData;
+    }
+}
+
+namespace {
+
+template <int R, int MODE, int TW, int TGY, int HALO>
+int launch_ring_one(const DevJob &job, int ntiles, hipStream_t s)
+{
+    using G = Geo<TW, TGY, HALO>;
+    const size_t lds = kTileOff + (size_t)G::TILE_BYTES + (size_t)(job.nrows + 1) * 1024;
+    auto kern = bf_ring_kernel<R, MODE, TW, TGY, HALO>;
+    static int configured = 0;
+    if (!configured) {
+        BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        configured = 1;
+    }
+    int dev = 0, cus = 0, per_cu = 0;
+    BF_HIP(hipGetDevice(&dev));
+    BF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    BF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G::NT, lds));
+    if (per_cu < 1) {
+        set_error("ring kernel does not fit on a CU");
+        return kInvalidData;
+    }
+    const int grid = std::min(ntiles, cus * per_cu);
+    if (grid < 1) return kOk;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(G::NT), lds, s, job, ntiles);
+    BF_HIP(hipGetLastError());
+    return kOk;
+}
+
+}  // namespace
+
+bool ring_available(int radius, int mode, int *config)
+{
+    switch (radius) {
+    case 15: *config = 0; return true;
+    case 3: case 5: case 7: case 9: *config = 1; return true;
+    default: (void)mode; return false;
+    }
+}
+
+int ring_rows(int radius, int mode)
+{
+    switch (radius * 2 + (mode == kModeRef ? 0 : 1)) {
+    case 30: return RingTaps<15, 0>().nrows;
+    case 31: return RingTaps<15, 1>().nrows;
+    case 6: return RingTaps<3, 0>().nrows;
+    case 7: return RingTaps<3, 1>().nrows;
+    case 10: return RingTaps<5, 0>().nrows;
+    case 11: return RingTaps<5, 1>().nrows;
+    case 14: return RingTaps<7, 0>().nrows;
+    case 15: return RingTaps<7, 1>().nrows;
+    case 18: return RingTaps<9, 0>().nrows;
+    case 19: return RingTaps<9, 1>().nrows;
+    default: return -1;
+    }
+}
+
+int launch_ring(const DevJob &job, int radius, int mode, int ntiles, hipStream_t s)
+{
+    const bool ref = mode == kModeRef;
+    switch (radius) {
+    case 15: return ref ? launch_ring_one<15, 0, 64, 16, 16>(job, ntiles, s)
+                        : launch_ring_one<15, 1, 64, 16, 16>(job, ntiles, s);
+    case 9: return ref ? launch_ring_one<9, 0, 64, 8, 12>(job, ntiles, s)
+                       : launch_ring_one<9, 1, 64, 8, 12>(job, ntiles, s);
+    case 7: return ref ? launch_ring_one<7, 0, 64, 8, 8>(job, ntiles, s)
+                       : launch_ring_one<7, 1, 64, 8, 8>(job, ntiles, s);
+    case 5: return ref ? launch_ring_one<5, 0, 64, 8, 8>(job, ntiles, s)
+                       : launch_ring_one<5, 1, 64, 8, 8>(job, ntiles, s);
+    case 3: return ref ? launch_ring_one<3, 0, 64, 8, 4>(job, ntiles, s)
+                       : launch_ring_one<3, 1, 64, 8, 4>(job, ntiles, s);
+    default: set_error("no unrolled kernel for this radius"); return kInvalidData;
     }
 }
 

@@ -57,6 +57,8 @@ struct Lut {
     uint32_t *d_taps = nullptr;      // tap variants (see bf_lut_create)
     float *d_gtap = nullptr;         // spatial weight per entry of d_taps (0 for null taps)
     float *d_rowg = nullptr;         // nrows_total
+    bool canonical = false;          // offsets = the GAUSS ring order, one weight per q
+    float *d_rowq = nullptr;         // canonical: weight of each distinct q, first-appearance order
     float *d_range = nullptr;        // 256
 };
 
@@ -114,6 +116,12 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut,
 // Fused-table rows needed by (lut, radius, mode), or -1 if the radius
 // exceeds the template.
 int fused_rows(const Lut *lut, int radius, int mode);
+
+// Unrolled kernels for the canonical ring template: which radii have one
+// (and the tile config they use), their fused-table row count, the launch.
+bool ring_available(int radius, int mode, int *config);
+int ring_rows(int radius, int mode);
+int launch_ring(const DevJob &job, int radius, int mode, int ntiles, hipStream_t s);
 
 Lut *lut_from_handle(bf_lut h);
 int current_device();
