@@ -162,6 +162,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy (torch copy_): "
                          "the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE counters")
@@ -240,7 +241,8 @@ def main() -> int:
     std_ms = e0.elapsed_time(e1) / nstd
 
     std_out = tout.cpu().numpy()     # exact STD result (the surrogate's target)
-    sur = surrogate_leg(args, it, img, tin, lut, std_out, std_ms, ms_per_step, stream)
+    sur = None if args.no_surrogate else surrogate_leg(args, it, img, tin, lut, std_out, std_ms,
+                                                       ms_per_step, stream)
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "exact_c3_traffic.json")

@@ -501,6 +501,7 @@ struct CheckArgs {
     int32_t *accum;              // optional running total of fallback tiles
     int32_t *tiles_accum;        // optional running total of tiles
     uint32_t salt;
+    int halo;                    // context rows/columns staged around a tile
 };
 
 __device__ __forceinline__ uint32_t hash32(uint32_t x)
@@ -516,33 +517,52 @@ __device__ __forceinline__ uint8_t to_u8_exact(float q)
     return (uint8_t)(int)q;
 }
 
+// One wave per 64 x 64 tile, one sample per lane.  The tile, its context
+// rows and columns (clamped to the input view, like the exact kernels) and
+// the range table are staged in LDS; each lane then runs the exact tap loop
+// in the exact kernels' order, so its value is the exact filter's.
+constexpr int kCheckHaloMax = 20;     // largest halo class
+constexpr int kCheckSpanMax = 64 + 2 * kCheckHaloMax;
+
 __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
 {
+    __shared__ uint8_t win[kCheckSpanMax * kCheckSpanMax];
+    __shared__ float rng[256];
     const int tile = blockIdx.x;
     const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const DevJob &j = a.job;
+    const int halo = a.halo, span = 64 + 2 * halo;
+    const int gx0 = tx * 64 - halo, gy0 = ty * 64 - halo;
+    for (int i = threadIdx.x; i < 256; i += 64) rng[i] = j.range[i];
+    for (int yy = 0; yy < span; yy++) {
+        int gy = gy0 + yy;
+        gy = gy < 0 ? 0 : (gy >= j.in_h ? j.in_h - 1 : gy);
+        const unsigned char *src = j.in + (int64_t)gy * j.in_pitch;
+        for (int xx = threadIdx.x; xx < span; xx += 64) {
+            int gx = gx0 + xx;
+            gx = gx < 0 ? 0 : (gx >= j.in_w ? j.in_w - 1 : gx);
+            win[yy * span + xx] = src[gx];
+        }
+    }
+    __syncthreads();
     const int cell = threadIdx.x;            // 8 x 8 cells of 8 x 8 pixels
     const uint32_t h = hash32((uint32_t)tile * 64u + (uint32_t)cell + a.salt);
     const int x = tx * 64 + (cell & 7) * 8 + (int)(h & 7);
     const int y = ty * 64 + (cell >> 3) * 8 + (int)((h >> 3) & 7);
-    const DevJob &j = a.job;
     bool bad = false;
     if (x < j.W && y < j.H) {
         const int y0 = j.mode == BF_MODE_REF ? y - (y & 3) : y;
-        auto T = [&](int xx, int yy) -> uint32_t {
-            xx = xx < 0 ? 0 : (xx >= j.in_w ? j.in_w - 1 : xx);
-            yy = yy < 0 ? 0 : (yy >= j.in_h ? j.in_h - 1 : yy);
-            return j.in[(int64_t)yy * j.in_pitch + xx];
-        };
-        const uint32_t c = T(x, y);
+        const int base = (y0 - gy0) * span + (x - gx0);
+        const uint32_t c = win[(y - gy0) * span + (x - gx0)];
         float t = 0.0f, s = 0.0f;
         for (int k = 0; k < j.ntaps_padded; k++) {
             const uint32_t info = j.taps[k];
             const int dy = (int)(int8_t)(info & 0xFFu), dx = (int)(int8_t)((info >> 8) & 0xFFu);
             const int row = (int)(info >> 16);
-            const uint32_t p = T(x + dx, y0 + dy);
+            const uint32_t p = win[base + dy * span + dx];
             const uint32_t d = p > c ? p - c : c - p;
             const float g = row < j.nrows ? a.rowg[row] : 0.0f;
-            const float f = g * j.range[d];
+            const float f = g * rng[d];
             t = t + f * (float)p;
             s = s + f;
         }
@@ -990,6 +1010,13 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     ca.accum = accum;
     ca.tiles_accum = tiles_accum;
     ca.salt = 0x5bd1e995u;
+    // make_job has checked every tap against this halo (REF rows may reach
+    // two further down, which the 4-row grouping of the sample row covers).
+    ca.halo = halo_class(radius);
+    if (ca.halo < 0 || ca.halo > kCheckHaloMax) {
+        set_error("radius outside the check kernel's halo");
+        return kInvalidData;
+    }
     hipLaunchKernelGGL(check_kernel, dim3(nt), dim3(64), 0, s, ca);
     BF_HIP(hipGetLastError());
     // exact fallback over the listed tiles

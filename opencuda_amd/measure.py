@@ -1,0 +1,154 @@
+"""Throughput of the batched and mixed workloads on the current GPU.
+
+    python -m opencuda_amd.measure c2 [--steps 5]     # batched persistent launch
+    python -m opencuda_amd.measure c4 [--steps 3]     # exact batch + surrogate, one graph
+
+c2 (256 images, 256..2048 per side, ROI 50..100 %, r in {3,5,9}, REF mode):
+one bf_batch launch (eager and replayed from a captured graph) against the
+same jobs issued as one bf_exact launch per image, all on one stream.
+
+c4 (512 images, 128..4096 per side, r in {3,7,15}, STD mode): the mixed
+graph (r >= 9 through the surrogate with sampled check and exact fallback)
+against one bf_batch launch that filters every image exactly.
+
+Each prints one JSON line; progress goes to stderr.  Inputs are resident in
+HBM before timing starts; times come from HIP events on the launch stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+from . import Batch, Job, Lut, MODE_REF, MODE_STD, exact, view
+from . import workloads as wl
+
+
+def _log(m):
+    print(f"[measure] {m}", file=sys.stderr, flush=True)
+
+
+def _flop(it, mode):
+    T = (2 * it.radius + 1) ** 2 + (1 if mode == MODE_REF else 0)
+    return it.pixels * (4 * T + 1)
+
+
+def _time(fn, steps, stream):
+    import torch
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def c2(steps: int = 5, mode: int = MODE_REF, tune_table: str | None = None):
+    import torch
+    items = wl.c2()
+    stream = torch.cuda.current_stream()
+    luts, tin, tout, jobs = {}, [], [], []
+    t0 = time.time()
+    for it in items:
+        key = (it.sigma_space, it.sigma_range)
+        if key not in luts:
+            luts[key] = Lut.from_sigmas(*key)
+        tin.append(torch.from_numpy(it.image()).cuda())
+        x1, y1, x2, y2 = it.roi
+        tout.append(torch.zeros((y2 - y1, x2 - x1), dtype=torch.uint8, device="cuda"))
+        jobs.append(Job(view(tin[-1], it.roi), view(tout[-1]), luts[key].handle, it.radius, mode))
+    _log(f"{len(items)} images uploaded in {time.time() - t0:.1f} s")
+    px = sum(it.pixels for it in items)
+    flop = sum(_flop(it, mode) for it in items)
+
+    def per_image():
+        for it, j in zip(items, jobs):
+            exact(j.inp, j.out, luts[(it.sigma_space, it.sigma_range)], it.radius, mode,
+                  stream=stream)
+
+    b = Batch(jobs)
+    for _ in range(2):
+        b.run(stream)
+        b.run_graph(stream)
+        per_image()
+    torch.cuda.synchronize()
+    ms_batch = _time(lambda: b.run(stream), steps, stream)
+    ms_graph = _time(lambda: b.run_graph(stream), steps, stream)
+    ms_loop = _time(per_image, steps, stream)
+    _log(f"batch {ms_batch:.3f} ms, graph {ms_graph:.3f} ms, per-image launches {ms_loop:.3f} ms")
+    tuned = None
+    if tune_table:
+        from . import autotune
+        n = autotune.load(tune_table)
+        bt = Batch(jobs)
+        bt.run_graph(stream)
+        torch.cuda.synchronize()
+        ms_tuned = _time(lambda: bt.run_graph(stream), steps, stream)
+        tuned = {"entries": n, "graph_ms": round(ms_tuned, 4),
+                 "graph_mpix_s": round(px / ms_tuned / 1e3, 1)}
+        _log(f"with the tuned table ({n} entries): graph {ms_tuned:.3f} ms")
+    by_r = {}
+    for it in items:
+        by_r[it.radius] = by_r.get(it.radius, 0) + 1
+    return {
+        "workload": "c2: 256 images 256..2048 per side, ROI 50..100 %, r in {3,5,9}, "
+                    f"{'REF' if mode == MODE_REF else 'STD'} mode",
+        "images": len(items), "images_per_radius": by_r, "pixels": px,
+        "batch_ms": round(ms_batch, 4), "graph_ms": round(ms_graph, 4),
+        "per_image_launches_ms": round(ms_loop, 4),
+        "batch_mpix_s": round(px / ms_batch / 1e3, 1),
+        "graph_mpix_s": round(px / ms_graph / 1e3, 1),
+        "per_image_mpix_s": round(px / ms_loop / 1e3, 1),
+        "batch_tflops": round(flop / ms_batch / 1e9, 2),
+        "batch_vs_per_image": round(ms_loop / ms_batch, 3),
+        "tuned": tuned,
+    }
+
+
+def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD):
+    import torch
+    from .mixed import measure
+    out, mx = measure(count, steps, mode, log=_log)
+    stream = torch.cuda.current_stream()
+    jobs = [Job(view(mx.tin[i], it.roi), view(mx.tout[i]),
+                mx.luts[(it.sigma_space, it.sigma_range)].handle, it.radius, mode)
+            for i, it in enumerate(mx.items)]
+    b = Batch(jobs)
+    b.run_graph(stream)
+    torch.cuda.synchronize()
+    ms_exact = _time(lambda: b.run_graph(stream), steps, stream)
+    sur_px = mx.sur_pixels
+    out.update({
+        "all_exact_graph_ms": round(ms_exact, 3),
+        "all_exact_mpix_s": round(mx.pixels / ms_exact / 1e3, 1),
+        "mixed_vs_all_exact": round(ms_exact / out["ms_per_graph"], 4),
+        "surrogate_pixels": sur_px,
+    })
+    _log(f"mixed {out['ms_per_graph']} ms, all exact {ms_exact:.3f} ms")
+    return out
+
+
+def _main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("workload", choices=["c2", "c4"])
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--count", type=int, default=512, help="c4: number of images")
+    ap.add_argument("--mode", choices=["ref", "std"], default=None)
+    ap.add_argument("--tune", default=None, help="c2: also time with this autotuner table")
+    args = ap.parse_args()
+    import torch
+    torch.cuda.set_device(0)
+    if args.workload == "c2":
+        mode = MODE_STD if args.mode == "std" else MODE_REF
+        res = c2(args.steps or 5, mode, args.tune)
+    else:
+        mode = MODE_REF if args.mode == "ref" else MODE_STD
+        res = c4(args.count, args.steps or 3, mode)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(_main())
