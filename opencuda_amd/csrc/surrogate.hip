@@ -579,6 +579,351 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
     if (threadIdx.x == 0 && a.tiles_accum) atomicAdd(a.tiles_accum, 1);
 }
 
+// ------------------------------------------------------- fused inference
+//
+// All six layers of the forward pass in one kernel, for inference: no
+// activation leaves the chip.  A workgroup owns a strip of FS_W output
+// columns and a band of rows and streams down the band.  At step t, layer l
+// (0..5) computes its row t - 2l; layer l's row r is read by layer l+1 at
+// steps r+2l+1 .. r+2l+3 (as its rows r-1 .. r+1), so each layer keeps a
+// ring of FS_RING = 4 rows in LDS.  Layer l's rows span FS_W + 2(5-l)
+// columns starting 5-l left of the strip: the context the layers above need.
+//
+// Eight waves, two per SIMD.  Wave w runs hidden layer 1 + w/2 on M-tiles
+// 2(w%2) and 2(w%2)+1 (16 pixels each) with both 16-channel N-tiles, so each
+// A fragment read from LDS feeds two MFMAs.  Waves 0..3 also run M-tile w of
+// the last layer and store the output; waves 4..7 run the first layer
+// (M-tile w-4, wave 4 also M-tile 4) and prefetch image rows.  Every wave
+// keeps its B operands in VGPRs for the whole band.  MFMAs per step: 45 for
+// waves 0..3, 38..40 for waves 4..7, i.e. 83..85 per SIMD.
+//
+// Image rows pass through a 64-row LDS ring: waves 4..7 load 16 rows into
+// registers 16 steps before the ring needs them (these waves issue no other
+// global memory operation, and the step barrier waits for LDS only, so the
+// loads overlap the MFMA work).
+//
+// Activations are 64-byte NHWC pixels in ring rows whose pitch is a
+// multiple of 16 pixels; the 16-byte chunk c of the pixel in column x sits
+// at chunk c ^ ((x >> 2) & 3), so the 16 consecutive pixels one
+// ds_read_b128 gathers fall into distinct banks, and every per-lane part of
+// an address is the same at every step.
+//
+// Operands, their order and every rounding are those of conv_kernel, so the
+// output equals the per-layer path's bit for bit.
+
+constexpr int FS_W = 56;                  // output columns per strip
+constexpr int FS_WAVES = 8;
+constexpr int FS_RING = 4;                // rows per layer ring
+constexpr int FS_IMG_ROWS = 64;           // image ring rows
+constexpr int FS_CHUNK = 16;              // image rows per prefetch
+constexpr int FS_PRE = 19;                // image rows loaded before the first step
+constexpr int FS_PF = 5;                  // prefetched bytes per thread of waves 4..7
+constexpr int fs_w(int l) { return FS_W + 2 * (NLAYERS - 1 - l); }     // 66 64 62 60 58 56
+constexpr int fs_pitch(int l) { return (fs_w(l) + 15) / 16 * 16; }     // 80 64 64 64 64 (pixels)
+constexpr int fs_off(int l) { return l == 0 ? 0 : fs_off(l - 1) + FS_RING * fs_pitch(l - 1) * 64; }
+constexpr int FS_IMG_W = fs_w(0) + 2;                                  // 68 image columns used
+constexpr int FS_IMG_PITCH = 96;          // >= 16 * 5 + 2: the first layer's padded M-tiles read in-row
+constexpr int FS_IMG_OFF = fs_off(NLAYERS - 1);
+constexpr int FS_LUT_OFF = FS_IMG_OFF + FS_IMG_ROWS * FS_IMG_PITCH;    // bf16(u / 255), u = 0..255
+constexpr int FS_DUMP = FS_LUT_OFF + 512;                              // masked-off lanes' writes
+constexpr int FS_LDS = FS_DUMP + 64;
+constexpr int FS_CHUNK_BYTES = FS_CHUNK * FS_IMG_W;                    // 1088
+static_assert(fs_w(1) <= 64 && fs_w(5) > 48 && fs_w(0) <= 80 && fs_w(0) > 64,
+              "four M-tiles per hidden/last layer, five for the first");
+static_assert(FS_CHUNK_BYTES <= FS_PF * 256, "a prefetch fits the registers of waves 4..7");
+static_assert(FS_LDS <= 160 * 1024, "LDS");
+
+struct FusedArgs {
+    const uint8_t *img;
+    int64_t img_pitch;
+    uint8_t *out;
+    int64_t out_pitch;
+    int H, W;
+    int band;                     // rows per band
+    int strips;                   // strips per band row
+    float sr;                     // sigma_range / 100
+    const __bf16 *fw[NLAYERS];    // packed forward B operands
+    const float *bias[NLAYERS];
+};
+
+typedef __attribute__((address_space(3))) unsigned char fs_lds_u8;
+typedef __attribute__((address_space(3))) __bf16 fs_lds_bf;
+typedef __attribute__((address_space(3))) bf16x8 fs_lds_b128;
+
+__device__ __forceinline__ bf16x8 fs_read_a(uint32_t addr) { return *(const fs_lds_b128 *)(size_t)addr; }
+__device__ __forceinline__ void fs_write_bf(uint32_t addr, __bf16 v) { *(fs_lds_bf *)(size_t)addr = v; }
+__device__ __forceinline__ __bf16 fs_read_bf(uint32_t addr) { return *(const fs_lds_bf *)(size_t)addr; }
+__device__ __forceinline__ uint8_t fs_read_u8(uint32_t addr) { return *(const fs_lds_u8 *)(size_t)addr; }
+__device__ __forceinline__ void fs_write_u8(uint32_t addr, uint8_t v) { *(fs_lds_u8 *)(size_t)addr = v; }
+
+// Byte offset, within a ring row, of 16-byte chunk `chunk` of column x.
+__device__ __forceinline__ uint32_t fs_col(int x, int chunk)
+{
+    return (uint32_t)(x * 64 + ((chunk ^ ((x >> 2) & 3)) << 4));
+}
+
+// LDS writes visible to the whole workgroup, without waiting for the
+// register prefetch of image rows still in flight.
+__device__ __forceinline__ void fs_step_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+__device__ __forceinline__ uint8_t fs_load_img(const FusedArgs &a, int row, int col)
+{
+    row = row < 0 ? 0 : (row >= a.H ? a.H - 1 : row);
+    col = col < 0 ? 0 : (col >= a.W ? a.W - 1 : col);
+    return a.img[(int64_t)row * a.img_pitch + col];
+}
+
+__global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int i16 = lane & 15, kq = lane >> 4;
+    const int strip = blockIdx.x % a.strips, bandi = blockIdx.x / a.strips;
+    const int x0 = strip * FS_W;
+    const int Y0 = bandi * a.band;
+    const int Y1 = min(Y0 + a.band, a.H);
+    const int H = a.H, W = a.W;
+    const int img_base = Y0 - 6;              // image row held in ring slot 0
+    const int img_x = x0 - 6;                 // image column of ring column 0
+    const int t_begin = Y0 - 5, t_end = Y1 + 10;
+    const bool lastw = wave < 4;              // waves 0..3: last layer; 4..7: first layer + prefetch
+    const int ptid = tid - 256;               // prefetch thread index (waves 4..7)
+
+    auto img_row = [&](int row) {
+        return (uint32_t)(FS_IMG_OFF + ((row - img_base) & (FS_IMG_ROWS - 1)) * FS_IMG_PITCH);
+    };
+
+    // ---- B operands and biases
+    const int L = 1 + (wave >> 1);            // hidden layer
+    const int m0 = 2 * (wave & 1);
+    const __bf16 *fw = L == 1 ? a.fw[1] : (L == 2 ? a.fw[2] : (L == 3 ? a.fw[3] : a.fw[4]));
+    const float *bl = L == 1 ? a.bias[1] : (L == 2 ? a.bias[2] : (L == 3 ? a.bias[3] : a.bias[4]));
+    bf16x8 bw[9][2];
+#pragma unroll
+    for (int s = 0; s < 9; s++)
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+            bw[s][nt] = *reinterpret_cast<const bf16x8 *>(
+                fw + ((size_t)(nt * 16 + i16) * 288 + s * 32 + 8 * kq));
+    const float hb0 = bl[i16], hb1 = bl[16 + i16];
+    // bx: the last layer's nine k-steps (waves 0..3) or the first layer's two N-tiles (4..7)
+    bf16x8 bx[9];
+    float xb0, xb1;
+    if (lastw) {
+        const __bf16 *fl = a.fw[NLAYERS - 1];
+#pragma unroll
+        for (int s = 0; s < 9; s++)
+            bx[s] = *reinterpret_cast<const bf16x8 *>(fl + ((size_t)i16 * 288 + s * 32 + 8 * kq));
+        xb0 = a.bias[NLAYERS - 1][0];
+        xb1 = 0.0f;
+    } else {
+        const __bf16 *f0 = a.fw[0];
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+            bx[nt] = *reinterpret_cast<const bf16x8 *>(f0 + ((size_t)(nt * 16 + i16) * 32 + 8 * kq));
+#pragma unroll
+        for (int s = 2; s < 9; s++) bx[s] = bx[0];
+        xb0 = a.bias[0][i16];
+        xb1 = a.bias[0][16 + i16];
+    }
+
+    // ---- per-lane, step-invariant address parts
+    // hidden layer L reads layer L-1: column c + 1 + dx, c = 16 m0 + i16 (+16 for the second M-tile)
+    uint32_t ha[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) ha[d] = fs_col(16 * m0 + i16 + d, kq);
+    // hidden layer L writes: column 16 m0 + 4 kq (+ 16 mm + rr), channel 16 nt + i16
+    uint32_t hw[2];
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++)
+        hw[nt] = fs_col(16 * m0 + 4 * kq, 2 * nt + (i16 >> 3)) + (i16 & 7) * 2;
+    const int h_in_pitch = fs_pitch(0) - (L == 1 ? 0 : 16);       // 80 for layer 0, else 64
+    const int h_win = fs_w(0) - 2 * (L - 1), h_wout = h_win - 2;   // widths of layers L-1, L
+    const uint32_t h_in_off = L == 1 ? fs_off(0) : (L == 2 ? fs_off(1) : (L == 3 ? fs_off(2) : fs_off(3)));
+    const uint32_t h_out_off = L == 1 ? fs_off(1) : (L == 2 ? fs_off(2) : (L == 3 ? fs_off(3) : fs_off(4)));
+    const int h_cx = x0 - (5 - L);
+    // last layer (waves 0..3) reads layer 4 at column 16 wave + i16 + dx + 1
+    uint32_t la[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) la[d] = fs_col(16 * (wave & 3) + i16 + d, kq);
+    // first layer (waves 4..7): this lane's four taps 4 kq + j (taps >= 9 are K padding)
+    int f_dy[4], f_dx[4];
+    bool f_tap[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int tap = 4 * kq + j;
+        f_tap[j] = tap < 9;
+        const int tp = tap < 9 ? tap : 4;
+        f_dy[j] = tp / 3 - 1;
+        f_dx[j] = tp % 3 - 1;
+    }
+
+    // ---- tables and the first FS_PRE image rows
+    if (tid < 256) fs_write_bf(FS_LUT_OFF + 2 * tid, to_bf((float)tid / 255.0f));
+    for (int e = tid; e < FS_PRE * FS_IMG_W; e += FS_WAVES * 64) {
+        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
+        const int row = t_begin - 1 + rr;
+        fs_write_u8(img_row(row) + cc, fs_load_img(a, row, img_x + cc));
+    }
+    fs_step_barrier();
+
+    uint8_t pf[FS_PF];                        // prefetched image bytes (waves 4..7)
+#pragma unroll
+    for (int j = 0; j < FS_PF; j++) pf[j] = 0;
+    for (int t = t_begin; t < t_end; t++) {
+        const int phase = (t - t_begin) & (FS_CHUNK - 1);
+        if (!lastw && phase == 0) {
+            // rows [t + 18, t + 34): written at step t + 15, first read at step t + 17
+            const int r0 = t - 1 + FS_PRE;
+#pragma unroll
+            for (int j = 0; j < FS_PF; j++) {
+                const int e = ptid + 256 * j;
+                if (e < FS_CHUNK_BYTES) {
+                    const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
+                    pf[j] = fs_load_img(a, r0 + rr, img_x + cc);
+                }
+            }
+        }
+
+        // ---- hidden layer L, row t - 2L
+        {
+            const int r = t - 2 * L;
+            if (r >= Y0 - (5 - L) && r < Y1 + (5 - L)) {
+                const bool row_in = r >= 0 && r < H;
+                f32x4 acc[2][2];
+#pragma unroll
+                for (int mm = 0; mm < 2; mm++)
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) acc[mm][nt] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
+                if (row_in) {
+#pragma unroll
+                    for (int s = 0; s < 9; s++) {
+                        const int dy = s / 3 - 1, dx = s % 3 - 1;
+                        const uint32_t rb = h_in_off + (uint32_t)(((r + dy) & (FS_RING - 1)) * h_in_pitch * 64);
+#pragma unroll
+                        for (int mm = 0; mm < 2; mm++) {
+                            const bf16x8 av = fs_read_a(rb + ha[dx + 1] + mm * 1024);
+#pragma unroll
+                            for (int nt = 0; nt < 2; nt++)
+                                acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[s][nt], acc[mm][nt], 0, 0, 0);
+                        }
+                    }
+                }
+                const uint32_t wb = h_out_off + (uint32_t)((r & (FS_RING - 1)) * 64 * 64);
+#pragma unroll
+                for (int mm = 0; mm < 2; mm++)
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int c = 16 * (m0 + mm) + 4 * kq + rr;
+                        const int gx = h_cx + c;
+                        const bool in = row_in && gx >= 0 && gx < W;
+                        const bool keep = c < h_wout;
+#pragma unroll
+                        for (int nt = 0; nt < 2; nt++) {
+                            float v = acc[mm][nt][rr] + (nt ? hb1 : hb0);
+                            v = v > 0.0f ? v : 0.0f;
+                            const uint32_t ad = wb + hw[nt] + (16 * mm + rr) * 64;
+                            fs_write_bf(keep ? ad : FS_DUMP + 2 * i16, to_bf(in ? v : 0.0f));
+                        }
+                    }
+            }
+        }
+
+        if (lastw) {
+            // ---- last layer, row t - 10, M-tile `wave`
+            const int r = t - 10;
+            if (r >= Y0 && r < Y1) {
+                f32x4 acc = (f32x4){ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+                for (int s = 0; s < 9; s++) {
+                    const int dy = s / 3 - 1, dx = s % 3 - 1;
+                    const uint32_t rb = fs_off(NLAYERS - 2) + (uint32_t)(((r + dy) & (FS_RING - 1)) * 64 * 64);
+                    const bf16x8 av = fs_read_a(rb + la[dx + 1]);
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bx[s], acc, 0, 0, 0);
+                }
+                if (i16 == 0) {
+                    const uint32_t ib = img_row(r);
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int c = 16 * wave + 4 * kq + rr;
+                        const int gx = x0 + c;
+                        if (c < FS_W && gx < W) {
+                            const uint8_t px = fs_read_u8(ib + c + 6);
+                            const float v = (float)px / 255.0f + (acc[rr] + xb0);
+                            const float q = rintf(v * 255.0f);
+                            a.out[(int64_t)r * a.out_pitch + gx] =
+                                (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
+                        }
+                    }
+                }
+            }
+        } else {
+            // ---- first layer, row t (>= t_begin = Y0 - 5 always)
+            const int r = t;
+            if (r < Y1 + 5) {
+                const bool row_in = r >= 0 && r < H;
+                const uint32_t wb = fs_off(0) + (uint32_t)((r & (FS_RING - 1)) * fs_pitch(0) * 64);
+                bool rowok[4];
+                uint32_t rb[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int gy = r + f_dy[j];
+                    rowok[j] = f_tap[j] && gy >= 0 && gy < H;
+                    rb[j] = img_row(gy);
+                }
+                const int mcount = wave == 4 ? 2 : 1;
+                for (int mi = 0; mi < mcount; mi++) {
+                    const int m = mi == 0 ? wave - 4 : 4;
+                    const int c = 16 * m + i16;
+                    bf16x8 av;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int col = c + 1 + f_dx[j];
+                        const int gx = img_x + col;
+                        const bool ok = rowok[j] && gx >= 0 && gx < W;
+                        const uint8_t u = fs_read_u8(rb[j] + col);
+                        const __bf16 v = fs_read_bf(FS_LUT_OFF + 2 * u);
+                        av[2 * j] = ok ? v : to_bf(0.0f);
+                        av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
+                    }
+                    f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bx[0], acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bx[1], acc1, 0, 0, 0);
+#pragma unroll
+                    for (int rr = 0; rr < 4; rr++) {
+                        const int cc = 16 * m + 4 * kq + rr;
+                        const int gx = x0 - 5 + cc;
+                        const bool in = row_in && gx >= 0 && gx < W;
+                        const bool keep = cc < fs_w(0);
+#pragma unroll
+                        for (int nt = 0; nt < 2; nt++) {
+                            float v = (nt ? acc1[rr] : acc0[rr]) + (nt ? xb1 : xb0);
+                            v = v > 0.0f ? v : 0.0f;
+                            const uint32_t ad = wb + fs_col(cc, 2 * nt + (i16 >> 3)) + (i16 & 7) * 2;
+                            fs_write_bf(keep ? ad : FS_DUMP + 2 * i16, to_bf(in ? v : 0.0f));
+                        }
+                    }
+                }
+            }
+            if (phase == FS_CHUNK - 1) {
+                const int r0 = t - FS_CHUNK + FS_PRE;            // the rows prefetched at phase 0
+#pragma unroll
+                for (int j = 0; j < FS_PF; j++) {
+                    const int e = ptid + 256 * j;
+                    if (e < FS_CHUNK_BYTES) {
+                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
+                        fs_write_u8(img_row(r0 + rr) + cc, pf[j]);
+                    }
+                }
+            }
+        }
+        fs_step_barrier();
+    }
+}
+
 __global__ void zero_ints(int32_t *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -657,6 +1002,41 @@ int launch_conv(const ConvArgs &a, hipStream_t s)
     const int64_t ntiles = (int64_t)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.N;
     const int grid = (int)std::min<int64_t>(ntiles, (int64_t)cus * 4);
     hipLaunchKernelGGL(conv_kernel<KIND>, dim3(grid), dim3(256), 0, s, a);
+    BF_HIP(hipGetLastError());
+    return kOk;
+}
+
+// Fused inference of one image (u8 in, u8 out): strips of FS_W columns x
+// bands of up to 256 rows, one workgroup of 10 waves each.
+int launch_fused(Net *net, const uint8_t *img, int64_t img_pitch, uint8_t *out, int64_t out_pitch,
+                 int H, int W, float sigma_range, hipStream_t s)
+{
+    static int configured = 0;
+    if (!configured) {
+        BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_infer_kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, FS_LDS));
+        configured = 1;
+    }
+    FusedArgs a{};
+    a.img = img;
+    a.img_pitch = img_pitch;
+    a.out = out;
+    a.out_pitch = out_pitch;
+    a.H = H;
+    a.W = W;
+    a.band = std::min(H, 256);
+    a.strips = (W + FS_W - 1) / FS_W;
+    a.sr = sigma_range / 100.0f;
+    for (int l = 0; l < NLAYERS; l++) {
+        a.fw[l] = net->fw[l];
+        a.bias[l] = net->params + poff(l) + wcount(l);
+    }
+    const int64_t grid = (int64_t)a.strips * ((H + a.band - 1) / a.band);
+    if (grid > 0x7FFFFFFF) {
+        set_error("image too large");
+        return kInvalidData;
+    }
+    hipLaunchKernelGGL(fused_infer_kernel, dim3((unsigned)grid), dim3(FS_WAVES * 64), FS_LDS, s, a);
     BF_HIP(hipGetLastError());
     return kOk;
 }
@@ -957,12 +1337,9 @@ static int infer_view(Net *net, const bf_image_view *in, const bf_image_view *ou
         set_error("empty view");
         return kInvalidData;
     }
-    int rc = ensure_ws(net, (size_t)W * H);
-    if (rc) return rc;
     *Wo = W;
     *Ho = H;
-    return forward(net, in->data, in->pitch, 0, nullptr, sigma_range, 1, H, W, nullptr, out->data,
-                   out->pitch, s);
+    return launch_fused(net, in->data, in->pitch, out->data, out->pitch, H, W, sigma_range, s);
 }
 
 int bf_sur_infer(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, float sigma_range,

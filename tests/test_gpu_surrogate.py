@@ -4,6 +4,8 @@
   (torch autograd on the CPU) with the same BF16 rounding points: weights,
   input channels and post-ReLU activations rounded to BF16;
 * training lowers the loss and is deterministic;
+* the fused inference kernel (all six layers in one launch) against the
+  per-layer kernels, bit for bit, on ragged, multi-band and pitched images;
 * the check-and-fallback path: tolerance -1 makes every tile exact
   (bit-exact to bf_exact), tolerance 255 leaves the pure surrogate output;
 * the trained network against the method without it (identity / untrained
@@ -158,6 +160,31 @@ def test_no_fallback_is_pure_surrogate(device):
     net.infer(oc.view(tin), oc.view(b), 30.0)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("shape,roi,sr", [
+    ((217, 300), None, 27.5),              # one band, last strip partial
+    ((530, 613), None, 40.0),              # three bands of 256 rows
+    ((40, 1000), (3, 5, 998, 37), 15.0),   # pitched ROI view, fewer rows than a band
+    ((13, 13), None, 30.0),                # smaller than one strip
+])
+def test_fused_inference_equals_per_layer_path(device, shape, roi, sr):
+    import torch
+    net = sg.Surrogate(seed=3)
+    if os.path.exists(sg.DEFAULT_WEIGHTS):
+        net.load(sg.DEFAULT_WEIGHTS)
+    H, W = shape
+    img = oc.synth_image(W, H, 20, 3, 80, 10.0, 5 + H)
+    tin = torch.from_numpy(img).cuda()
+    x1, y1, x2, y2 = roi or (0, 0, W, H)
+    out = torch.full((y2 - y1, x2 - x1), 77, dtype=torch.uint8, device="cuda")
+    net.infer(oc.view(tin, roi), oc.view(out), sr)
+    crop = tin[y1:y2, x1:x2].contiguous()
+    sigma = torch.tensor([sr], dtype=torch.float32, device="cuda")
+    pred, _, _ = net.forward_backward(crop[None], None, sigma)
+    ref = torch.clamp(torch.round(pred[0] * 255.0), 0, 255).to(torch.uint8)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
 
 
 def _within(a, b, tol=2):
