@@ -602,6 +602,10 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
 // global memory operation, and the step barrier waits for LDS only, so the
 // loads overlap the MFMA work).
 //
+// The MFMAs compute the transposed product (weights as the A operand,
+// activations as B), so a lane ends with four consecutive channels of one
+// pixel and stores them with one 8-byte write.
+//
 // Activations are 64-byte NHWC pixels in ring rows whose pitch is a
 // multiple of 16 pixels; the 16-byte chunk c of the pixel in column x sits
 // at chunk c ^ ((x >> 2) & 3), so the 16 consecutive pixels one
@@ -649,9 +653,26 @@ struct FusedArgs {
 typedef __attribute__((address_space(3))) unsigned char fs_lds_u8;
 typedef __attribute__((address_space(3))) __bf16 fs_lds_bf;
 typedef __attribute__((address_space(3))) bf16x8 fs_lds_b128;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) bf16x4 fs_lds_b64;
 
 __device__ __forceinline__ bf16x8 fs_read_a(uint32_t addr) { return *(const fs_lds_b128 *)(size_t)addr; }
 __device__ __forceinline__ void fs_write_bf(uint32_t addr, __bf16 v) { *(fs_lds_bf *)(size_t)addr = v; }
+__device__ __forceinline__ void fs_write_b64(uint32_t addr, bf16x4 v) { *(fs_lds_b64 *)(size_t)addr = v; }
+
+// Bias, ReLU and BF16 rounding of the four consecutive channels a lane
+// holds after a transposed MFMA; zero outside the image.
+__device__ __forceinline__ bf16x4 fs_act4(f32x4 acc, const float *b, bool in)
+{
+    bf16x4 o;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        float v = acc[r] + b[r];
+        v = v > 0.0f ? v : 0.0f;
+        o[r] = to_bf(in ? v : 0.0f);
+    }
+    return o;
+}
 __device__ __forceinline__ __bf16 fs_read_bf(uint32_t addr) { return *(const fs_lds_bf *)(size_t)addr; }
 __device__ __forceinline__ uint8_t fs_read_u8(uint32_t addr) { return *(const fs_lds_u8 *)(size_t)addr; }
 __device__ __forceinline__ void fs_write_u8(uint32_t addr, uint8_t v) { *(fs_lds_u8 *)(size_t)addr = v; }
@@ -708,17 +729,23 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         for (int nt = 0; nt < 2; nt++)
             bw[s][nt] = *reinterpret_cast<const bf16x8 *>(
                 fw + ((size_t)(nt * 16 + i16) * 288 + s * 32 + 8 * kq));
-    const float hb0 = bl[i16], hb1 = bl[16 + i16];
+    float hb[2][4];                           // bias of channels 16 nt + 4 kq + r
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) hb[nt][r] = bl[16 * nt + 4 * kq + r];
     // bx: the last layer's nine k-steps (waves 0..3) or the first layer's two N-tiles (4..7)
     bf16x8 bx[9];
-    float xb0, xb1;
+    float xb[2][4];
     if (lastw) {
         const __bf16 *fl = a.fw[NLAYERS - 1];
 #pragma unroll
         for (int s = 0; s < 9; s++)
             bx[s] = *reinterpret_cast<const bf16x8 *>(fl + ((size_t)i16 * 288 + s * 32 + 8 * kq));
-        xb0 = a.bias[NLAYERS - 1][0];
-        xb1 = 0.0f;
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) xb[nt][r] = a.bias[NLAYERS - 1][0];
     } else {
         const __bf16 *f0 = a.fw[0];
 #pragma unroll
@@ -726,8 +753,10 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
             bx[nt] = *reinterpret_cast<const bf16x8 *>(f0 + ((size_t)(nt * 16 + i16) * 32 + 8 * kq));
 #pragma unroll
         for (int s = 2; s < 9; s++) bx[s] = bx[0];
-        xb0 = a.bias[0][i16];
-        xb1 = a.bias[0][16 + i16];
+#pragma unroll
+        for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) xb[nt][r] = a.bias[0][16 * nt + 4 * kq + r];
     }
 
     // ---- per-lane, step-invariant address parts
@@ -735,11 +764,11 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     uint32_t ha[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) ha[d] = fs_col(16 * m0 + i16 + d, kq);
-    // hidden layer L writes: column 16 m0 + 4 kq (+ 16 mm + rr), channel 16 nt + i16
+    // hidden layer L writes column 16 m0 + i16 (+16 for the second M-tile),
+    // channels 16 nt + 4 kq .. +3: 8 bytes at byte 8 (kq & 1) of chunk 2 nt + kq / 2
     uint32_t hw[2];
 #pragma unroll
-    for (int nt = 0; nt < 2; nt++)
-        hw[nt] = fs_col(16 * m0 + 4 * kq, 2 * nt + (i16 >> 3)) + (i16 & 7) * 2;
+    for (int nt = 0; nt < 2; nt++) hw[nt] = fs_col(16 * m0 + i16, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
     const int h_in_pitch = fs_pitch(0) - (L == 1 ? 0 : 16);       // 80 for layer 0, else 64
     const int h_win = fs_w(0) - 2 * (L - 1), h_wout = h_win - 2;   // widths of layers L-1, L
     const uint32_t h_in_off = L == 1 ? fs_off(0) : (L == 2 ? fs_off(1) : (L == 3 ? fs_off(2) : fs_off(3)));
@@ -799,36 +828,47 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 #pragma unroll
                     for (int nt = 0; nt < 2; nt++) acc[mm][nt] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
                 if (row_in) {
+                    // A fragments of k-step s+2 are read while the MFMAs of
+                    // k-step s run (three buffers; the scheduling barriers keep
+                    // the reads ahead of the MFMAs that do not need them).
+                    uint32_t rbs[3];
+#pragma unroll
+                    for (int d = 0; d < 3; d++)
+                        rbs[d] = h_in_off + (uint32_t)(((r + d - 1) & (FS_RING - 1)) * h_in_pitch * 64);
+                    bf16x8 av[3][2];
+#pragma unroll
+                    for (int s = 0; s < 2; s++)
+#pragma unroll
+                        for (int mm = 0; mm < 2; mm++)
+                            av[s][mm] = fs_read_a(rbs[s / 3] + ha[s % 3] + mm * 1024);
 #pragma unroll
                     for (int s = 0; s < 9; s++) {
-                        const int dy = s / 3 - 1, dx = s % 3 - 1;
-                        const uint32_t rb = h_in_off + (uint32_t)(((r + dy) & (FS_RING - 1)) * h_in_pitch * 64);
+                        if (s + 2 < 9) {
 #pragma unroll
-                        for (int mm = 0; mm < 2; mm++) {
-                            const bf16x8 av = fs_read_a(rb + ha[dx + 1] + mm * 1024);
+                            for (int mm = 0; mm < 2; mm++)
+                                av[(s + 2) % 3][mm] = fs_read_a(rbs[(s + 2) / 3] + ha[(s + 2) % 3] + mm * 1024);
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int mm = 0; mm < 2; mm++)
 #pragma unroll
                             for (int nt = 0; nt < 2; nt++)
-                                acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[s][nt], acc[mm][nt], 0, 0, 0);
-                        }
+                                acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[s][nt], av[s % 3][mm], acc[mm][nt], 0, 0, 0);
+                        __builtin_amdgcn_sched_barrier(0);
                     }
                 }
                 const uint32_t wb = h_out_off + (uint32_t)((r & (FS_RING - 1)) * 64 * 64);
 #pragma unroll
-                for (int mm = 0; mm < 2; mm++)
+                for (int mm = 0; mm < 2; mm++) {
+                    const int c = 16 * (m0 + mm) + i16;
+                    const int gx = h_cx + c;
+                    const bool in = row_in && gx >= 0 && gx < W;
+                    const bool keep = c < h_wout;
 #pragma unroll
-                    for (int rr = 0; rr < 4; rr++) {
-                        const int c = 16 * (m0 + mm) + 4 * kq + rr;
-                        const int gx = h_cx + c;
-                        const bool in = row_in && gx >= 0 && gx < W;
-                        const bool keep = c < h_wout;
-#pragma unroll
-                        for (int nt = 0; nt < 2; nt++) {
-                            float v = acc[mm][nt][rr] + (nt ? hb1 : hb0);
-                            v = v > 0.0f ? v : 0.0f;
-                            const uint32_t ad = wb + hw[nt] + (16 * mm + rr) * 64;
-                            fs_write_bf(keep ? ad : FS_DUMP + 2 * i16, to_bf(in ? v : 0.0f));
-                        }
-                    }
+                    for (int nt = 0; nt < 2; nt++)
+                        fs_write_b64(keep ? wb + hw[nt] + mm * 1024 : FS_DUMP + 8 * (lane & 7),
+                                     fs_act4(acc[mm][nt], hb[nt], in));
+                }
             }
         }
 
@@ -837,27 +877,25 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
             const int r = t - 10;
             if (r >= Y0 && r < Y1) {
                 f32x4 acc = (f32x4){ 0.f, 0.f, 0.f, 0.f };
+                uint32_t rbs[3];
 #pragma unroll
-                for (int s = 0; s < 9; s++) {
-                    const int dy = s / 3 - 1, dx = s % 3 - 1;
-                    const uint32_t rb = fs_off(NLAYERS - 2) + (uint32_t)(((r + dy) & (FS_RING - 1)) * 64 * 64);
-                    const bf16x8 av = fs_read_a(rb + la[dx + 1]);
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bx[s], acc, 0, 0, 0);
-                }
-                if (i16 == 0) {
-                    const uint32_t ib = img_row(r);
+                for (int d = 0; d < 3; d++)
+                    rbs[d] = fs_off(NLAYERS - 2) + (uint32_t)(((r + d - 1) & (FS_RING - 1)) * 64 * 64);
+                bf16x8 av[9];
 #pragma unroll
-                    for (int rr = 0; rr < 4; rr++) {
-                        const int c = 16 * wave + 4 * kq + rr;
-                        const int gx = x0 + c;
-                        if (c < FS_W && gx < W) {
-                            const uint8_t px = fs_read_u8(ib + c + 6);
-                            const float v = (float)px / 255.0f + (acc[rr] + xb0);
-                            const float q = rintf(v * 255.0f);
-                            a.out[(int64_t)r * a.out_pitch + gx] =
-                                (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
-                        }
-                    }
+                for (int s = 0; s < 9; s++) av[s] = fs_read_a(rbs[s / 3] + la[s % 3]);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int s = 0; s < 9; s++)
+                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[s], av[s], acc, 0, 0, 0);
+                // output channel 0 of pixel 16 wave + i16 is acc[0] of the lanes with kq = 0
+                const int c = 16 * wave + i16;
+                const int gx = x0 + c;
+                if (kq == 0 && c < FS_W && gx < W) {
+                    const uint8_t px = fs_read_u8(img_row(r) + c + 6);
+                    const float v = (float)px / 255.0f + (acc[0] + xb[0][0]);
+                    const float q = rintf(v * 255.0f);
+                    a.out[(int64_t)r * a.out_pitch + gx] = (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
                 }
             }
         } else {
@@ -890,21 +928,15 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                         av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
                     }
                     f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bx[0], acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bx[1], acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[0], av, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[1], av, acc1, 0, 0, 0);
+                    const int gx = x0 - 5 + c;
+                    const bool in = row_in && gx >= 0 && gx < W;
+                    const bool keep = c < fs_w(0);
 #pragma unroll
-                    for (int rr = 0; rr < 4; rr++) {
-                        const int cc = 16 * m + 4 * kq + rr;
-                        const int gx = x0 - 5 + cc;
-                        const bool in = row_in && gx >= 0 && gx < W;
-                        const bool keep = cc < fs_w(0);
-#pragma unroll
-                        for (int nt = 0; nt < 2; nt++) {
-                            float v = (nt ? acc1[rr] : acc0[rr]) + (nt ? xb1 : xb0);
-                            v = v > 0.0f ? v : 0.0f;
-                            const uint32_t ad = wb + fs_col(cc, 2 * nt + (i16 >> 3)) + (i16 & 7) * 2;
-                            fs_write_bf(keep ? ad : FS_DUMP + 2 * i16, to_bf(in ? v : 0.0f));
-                        }
+                    for (int nt = 0; nt < 2; nt++) {
+                        const uint32_t ad = wb + fs_col(c, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
+                        fs_write_b64(keep ? ad : FS_DUMP + 8 * (lane & 7), fs_act4(nt ? acc1 : acc0, xb[nt], in));
                     }
                 }
             }

@@ -2,6 +2,7 @@
 
     python -m opencuda_amd.measure c2 [--steps 5]     # batched persistent launch
     python -m opencuda_amd.measure c4 [--steps 3]     # exact batch + surrogate, one graph
+    python -m opencuda_amd.measure c3-surrogate       # the surrogate on the 8192^2 image
 
 c2 (256 images, 256..2048 per side, ROI 50..100 %, r in {3,5,9}, REF mode):
 one bf_batch launch (eager and replayed from a captured graph) against the
@@ -10,6 +11,9 @@ same jobs issued as one bf_exact launch per image, all on one stream.
 c4 (512 images, 128..4096 per side, r in {3,7,15}, STD mode): the mixed
 graph (r >= 9 through the surrogate with sampled check and exact fallback)
 against one bf_batch launch that filters every image exactly.
+
+c3-surrogate (8192^2, r=15, STD target): fused BF16 inference alone, then
+inference + sampled exact check + exact fallback of failing 64x64 tiles.
 
 Each prints one JSON line; progress goes to stderr.  Inputs are resident in
 HBM before timing starts; times come from HIP events on the launch stream.
@@ -130,9 +134,41 @@ def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD):
     return out
 
 
+def c3_surrogate(steps: int = 5):
+    import torch
+    from .surrogate import DEFAULT_WEIGHTS, Surrogate
+    (it,) = wl.c3()
+    img = it.image()
+    tin = torch.from_numpy(img).cuda()
+    tout = torch.zeros_like(tin)
+    lut = Lut.from_sigmas(it.sigma_space, it.sigma_range)
+    net = Surrogate(seed=1)
+    net.load(DEFAULT_WEIGHTS)
+    stream = torch.cuda.current_stream()
+    vin, vout = view(tin), view(tout)
+    for _ in range(2):
+        net.infer(vin, vout, it.sigma_range, stream=stream)
+    torch.cuda.synchronize()
+    ms_inf = _time(lambda: net.infer(vin, vout, it.sigma_range, stream=stream), steps, stream)
+    total, fb = net.filter(vin, vout, lut, it.radius, MODE_STD, it.sigma_range, 2, stream=stream)
+    ms_filt = _time(lambda: net.filter(vin, vout, lut, it.radius, MODE_STD, it.sigma_range, 2,
+                                       stream=stream, counts=False), steps, stream)
+    px = it.pixels
+    _log(f"inference {ms_inf:.3f} ms, with check and fallback {ms_filt:.3f} ms "
+         f"({fb}/{total} tiles fell back)")
+    return {
+        "workload": "c3: 8192x8192, r=15, sigma_range=30, surrogate (STD target)",
+        "pixels": px, "inference_ms": round(ms_inf, 4),
+        "inference_mpix_s": round(px / ms_inf / 1e3, 1),
+        "inference_tflops_bf16": round(75456.0 * px / ms_inf / 1e9, 1),
+        "filter_ms": round(ms_filt, 4), "filter_mpix_s": round(px / ms_filt / 1e3, 1),
+        "tiles": total, "tiles_fallback": fb,
+    }
+
+
 def _main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["c2", "c4"])
+    ap.add_argument("workload", choices=["c2", "c4", "c3-surrogate"])
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--count", type=int, default=512, help="c4: number of images")
     ap.add_argument("--mode", choices=["ref", "std"], default=None)
@@ -140,7 +176,9 @@ def _main() -> int:
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
-    if args.workload == "c2":
+    if args.workload == "c3-surrogate":
+        res = c3_surrogate(args.steps or 5)
+    elif args.workload == "c2":
         mode = MODE_STD if args.mode == "std" else MODE_REF
         res = c2(args.steps or 5, mode, args.tune)
     else:
