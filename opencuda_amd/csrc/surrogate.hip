@@ -659,23 +659,24 @@ typedef __attribute__((address_space(3))) bf16x4 fs_lds_b64;
 __device__ __forceinline__ bf16x8 fs_read_a(uint32_t addr) { return *(const fs_lds_b128 *)(size_t)addr; }
 __device__ __forceinline__ void fs_write_bf(uint32_t addr, __bf16 v) { *(fs_lds_bf *)(size_t)addr = v; }
 __device__ __forceinline__ void fs_write_b64(uint32_t addr, bf16x4 v) { *(fs_lds_b64 *)(size_t)addr = v; }
-
-// Bias, ReLU and BF16 rounding of the four consecutive channels a lane
-// holds after a transposed MFMA; zero outside the image.
-__device__ __forceinline__ bf16x4 fs_act4(f32x4 acc, const float *b, bool in)
-{
-    bf16x4 o;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        float v = acc[r] + b[r];
-        v = v > 0.0f ? v : 0.0f;
-        o[r] = to_bf(in ? v : 0.0f);
-    }
-    return o;
-}
 __device__ __forceinline__ __bf16 fs_read_bf(uint32_t addr) { return *(const fs_lds_bf *)(size_t)addr; }
 __device__ __forceinline__ uint8_t fs_read_u8(uint32_t addr) { return *(const fs_lds_u8 *)(size_t)addr; }
 __device__ __forceinline__ void fs_write_u8(uint32_t addr, uint8_t v) { *(fs_lds_u8 *)(size_t)addr = v; }
+
+// Bias, ReLU and BF16 rounding of the four consecutive channels a lane
+// holds after a transposed MFMA; zero outside the image.  (fmaxf and the
+// per-layer kernels' v > 0 ? v : 0 differ only in the sign of a zero, which
+// no later sum, bias or rounding can observe.)
+__device__ __forceinline__ bf16x4 fs_act4(f32x4 acc, const float *b, bool in)
+{
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    bf16x4 o;
+#pragma unroll
+    for (int r = 0; r < 4; r++) o[r] = to_bf(fmaxf(acc[r] + b[r], 0.0f));
+    u32x2 bits = __builtin_bit_cast(u32x2, o);
+    bits = in ? bits : (u32x2){ 0u, 0u };
+    return __builtin_bit_cast(bf16x4, bits);
+}
 
 // Byte offset, within a ring row, of 16-byte chunk `chunk` of column x.
 __device__ __forceinline__ uint32_t fs_col(int x, int chunk)
@@ -817,6 +818,61 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
             }
         }
 
+        if (!lastw) {
+            // ---- first layer, row t (>= t_begin = Y0 - 5 always); waves 4..7 do
+            // this VALU-heavy work while waves 0..3 run their hidden MFMAs
+            const int r = t;
+            if (r < Y1 + 5) {
+                const bool row_in = r >= 0 && r < H;
+                const uint32_t wb = fs_off(0) + (uint32_t)((r & (FS_RING - 1)) * fs_pitch(0) * 64);
+                bool rowok[4];
+                uint32_t rb[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int gy = r + f_dy[j];
+                    rowok[j] = f_tap[j] && gy >= 0 && gy < H;
+                    rb[j] = img_row(gy);
+                }
+                const int mcount = wave == 4 ? 2 : 1;
+                for (int mi = 0; mi < mcount; mi++) {
+                    const int m = mi == 0 ? wave - 4 : 4;
+                    const int c = 16 * m + i16;
+                    bf16x8 av;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int col = c + 1 + f_dx[j];
+                        const int gx = img_x + col;
+                        const bool ok = rowok[j] && gx >= 0 && gx < W;
+                        const uint8_t u = fs_read_u8(rb[j] + col);
+                        const __bf16 v = fs_read_bf(FS_LUT_OFF + 2 * u);
+                        av[2 * j] = ok ? v : to_bf(0.0f);
+                        av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
+                    }
+                    f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[0], av, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[1], av, acc1, 0, 0, 0);
+                    const int gx = x0 - 5 + c;
+                    const bool in = row_in && gx >= 0 && gx < W;
+                    const bool keep = c < fs_w(0);
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++) {
+                        const uint32_t ad = wb + fs_col(c, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
+                        fs_write_b64(keep ? ad : FS_DUMP + 8 * (lane & 7), fs_act4(nt ? acc1 : acc0, xb[nt], in));
+                    }
+                }
+            }
+            if (phase == FS_CHUNK - 1) {
+                const int r0 = t - FS_CHUNK + FS_PRE;            // the rows prefetched at phase 0
+#pragma unroll
+                for (int j = 0; j < FS_PF; j++) {
+                    const int e = ptid + 256 * j;
+                    if (e < FS_CHUNK_BYTES) {
+                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
+                        fs_write_u8(img_row(r0 + rr) + cc, pf[j]);
+                    }
+                }
+            }
+        }
         // ---- hidden layer L, row t - 2L
         {
             const int r = t - 2 * L;
@@ -896,59 +952,6 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                     const float v = (float)px / 255.0f + (acc[0] + xb[0][0]);
                     const float q = rintf(v * 255.0f);
                     a.out[(int64_t)r * a.out_pitch + gx] = (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
-                }
-            }
-        } else {
-            // ---- first layer, row t (>= t_begin = Y0 - 5 always)
-            const int r = t;
-            if (r < Y1 + 5) {
-                const bool row_in = r >= 0 && r < H;
-                const uint32_t wb = fs_off(0) + (uint32_t)((r & (FS_RING - 1)) * fs_pitch(0) * 64);
-                bool rowok[4];
-                uint32_t rb[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int gy = r + f_dy[j];
-                    rowok[j] = f_tap[j] && gy >= 0 && gy < H;
-                    rb[j] = img_row(gy);
-                }
-                const int mcount = wave == 4 ? 2 : 1;
-                for (int mi = 0; mi < mcount; mi++) {
-                    const int m = mi == 0 ? wave - 4 : 4;
-                    const int c = 16 * m + i16;
-                    bf16x8 av;
-#pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int col = c + 1 + f_dx[j];
-                        const int gx = img_x + col;
-                        const bool ok = rowok[j] && gx >= 0 && gx < W;
-                        const uint8_t u = fs_read_u8(rb[j] + col);
-                        const __bf16 v = fs_read_bf(FS_LUT_OFF + 2 * u);
-                        av[2 * j] = ok ? v : to_bf(0.0f);
-                        av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
-                    }
-                    f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[0], av, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[1], av, acc1, 0, 0, 0);
-                    const int gx = x0 - 5 + c;
-                    const bool in = row_in && gx >= 0 && gx < W;
-                    const bool keep = c < fs_w(0);
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++) {
-                        const uint32_t ad = wb + fs_col(c, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
-                        fs_write_b64(keep ? ad : FS_DUMP + 8 * (lane & 7), fs_act4(nt ? acc1 : acc0, xb[nt], in));
-                    }
-                }
-            }
-            if (phase == FS_CHUNK - 1) {
-                const int r0 = t - FS_CHUNK + FS_PRE;            // the rows prefetched at phase 0
-#pragma unroll
-                for (int j = 0; j < FS_PF; j++) {
-                    const int e = ptid + 256 * j;
-                    if (e < FS_CHUNK_BYTES) {
-                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-                        fs_write_u8(img_row(r0 + rr) + cc, pf[j]);
-                    }
                 }
             }
         }
