@@ -592,12 +592,14 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
 // Eight waves, two per SIMD.  Wave w runs hidden layer 1 + w/2 on M-tiles
 // 2(w%2) and 2(w%2)+1 (16 pixels each) with both 16-channel N-tiles, so each
 // A fragment read from LDS feeds two MFMAs.  Waves 0..3 also run M-tile w of
-// the last layer and store the output; waves 4..7 run the first layer
-// (M-tile w-4, wave 4 also M-tile 4) and prefetch image rows.  Every wave
-// keeps its B operands in VGPRs for the whole band.  MFMAs per step: 45 for
-// waves 0..3, 38..40 for waves 4..7, i.e. 83..85 per SIMD.
+// the last layer and store the output; waves 4..7 run M-tile w-4 of the
+// first layer (before their hidden MFMAs, so this VALU-heavy work overlaps
+// the other waves' MFMAs) and prefetch image rows.  Every wave keeps its B
+// operands in VGPRs for the whole band.  MFMAs per step: 45 for waves 0..3,
+// 38 for waves 4..7, i.e. 83 per SIMD.
 //
-// Image rows pass through a 64-row LDS ring: waves 4..7 load 16 rows into
+// Image rows pass through a 64-row LDS ring, as u8 (the residual) and as
+// BF16 u/255 (the first layer's input): waves 4..7 load 16 rows into
 // registers 16 steps before the ring needs them (these waves issue no other
 // global memory operation, and the step barrier waits for LDS only, so the
 // loads overlap the MFMA work).
@@ -615,25 +617,26 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
 // Operands, their order and every rounding are those of conv_kernel, so the
 // output equals the per-layer path's bit for bit.
 
-constexpr int FS_W = 56;                  // output columns per strip
+constexpr int FS_W = 54;                  // output columns per strip (the first layer's 64 = 4 M-tiles)
 constexpr int FS_WAVES = 8;
 constexpr int FS_RING = 4;                // rows per layer ring
 constexpr int FS_IMG_ROWS = 64;           // image ring rows
 constexpr int FS_CHUNK = 16;              // image rows per prefetch
 constexpr int FS_PRE = 19;                // image rows loaded before the first step
 constexpr int FS_PF = 5;                  // prefetched bytes per thread of waves 4..7
-constexpr int fs_w(int l) { return FS_W + 2 * (NLAYERS - 1 - l); }     // 66 64 62 60 58 56
-constexpr int fs_pitch(int l) { return (fs_w(l) + 15) / 16 * 16; }     // 80 64 64 64 64 (pixels)
+constexpr int fs_w(int l) { return FS_W + 2 * (NLAYERS - 1 - l); }     // 64 62 60 58 56 54
+constexpr int fs_pitch(int l) { return (fs_w(l) + 15) / 16 * 16; }     // 64 (pixels)
 constexpr int fs_off(int l) { return l == 0 ? 0 : fs_off(l - 1) + FS_RING * fs_pitch(l - 1) * 64; }
-constexpr int FS_IMG_W = fs_w(0) + 2;                                  // 68 image columns used
-constexpr int FS_IMG_PITCH = 96;          // >= 16 * 5 + 2: the first layer's padded M-tiles read in-row
-constexpr int FS_IMG_OFF = fs_off(NLAYERS - 1);
-constexpr int FS_LUT_OFF = FS_IMG_OFF + FS_IMG_ROWS * FS_IMG_PITCH;    // bf16(u / 255), u = 0..255
+constexpr int FS_IMG_W = fs_w(0) + 2;                                  // 66 image columns used
+constexpr int FS_IMG_PITCH = 80;          // image ring row pitch (elements)
+constexpr int FS_IMG_OFF = fs_off(NLAYERS - 1);                        // u8 image ring (the residual)
+constexpr int FS_IMGB_OFF = FS_IMG_OFF + FS_IMG_ROWS * FS_IMG_PITCH;  // BF16 u/255 ring (first layer)
+constexpr int FS_LUT_OFF = FS_IMGB_OFF + FS_IMG_ROWS * FS_IMG_PITCH * 2;  // bf16(u / 255), u = 0..255
 constexpr int FS_DUMP = FS_LUT_OFF + 512;                              // masked-off lanes' writes
 constexpr int FS_LDS = FS_DUMP + 64;
-constexpr int FS_CHUNK_BYTES = FS_CHUNK * FS_IMG_W;                    // 1088
-static_assert(fs_w(1) <= 64 && fs_w(5) > 48 && fs_w(0) <= 80 && fs_w(0) > 64,
-              "four M-tiles per hidden/last layer, five for the first");
+constexpr int FS_CHUNK_BYTES = FS_CHUNK * FS_IMG_W;                    // 1056
+static_assert(fs_w(0) == 64 && fs_w(5) > 48, "four M-tiles per layer");
+static_assert(FS_IMG_PITCH >= FS_IMG_W, "image ring rows");
 static_assert(FS_CHUNK_BYTES <= FS_PF * 256, "a prefetch fits the registers of waves 4..7");
 static_assert(FS_LDS <= 160 * 1024, "LDS");
 
@@ -714,9 +717,8 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const bool lastw = wave < 4;              // waves 0..3: last layer; 4..7: first layer + prefetch
     const int ptid = tid - 256;               // prefetch thread index (waves 4..7)
 
-    auto img_row = [&](int row) {
-        return (uint32_t)(FS_IMG_OFF + ((row - img_base) & (FS_IMG_ROWS - 1)) * FS_IMG_PITCH);
-    };
+    // element index of image row `row` (>= img_base) in the image rings
+    auto img_idx = [&](int row) { return (uint32_t)(((row - img_base) & (FS_IMG_ROWS - 1)) * FS_IMG_PITCH); };
 
     // ---- B operands and biases
     const int L = 1 + (wave >> 1);            // hidden layer
@@ -770,7 +772,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     uint32_t hw[2];
 #pragma unroll
     for (int nt = 0; nt < 2; nt++) hw[nt] = fs_col(16 * m0 + i16, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
-    const int h_in_pitch = fs_pitch(0) - (L == 1 ? 0 : 16);       // 80 for layer 0, else 64
+    const int h_in_pitch = fs_pitch(0);                            // every ring row: 64 pixels
     const int h_win = fs_w(0) - 2 * (L - 1), h_wout = h_win - 2;   // widths of layers L-1, L
     const uint32_t h_in_off = L == 1 ? fs_off(0) : (L == 2 ? fs_off(1) : (L == 3 ? fs_off(2) : fs_off(3)));
     const uint32_t h_out_off = L == 1 ? fs_off(1) : (L == 2 ? fs_off(2) : (L == 3 ? fs_off(3) : fs_off(4)));
@@ -793,10 +795,14 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 
     // ---- tables and the first FS_PRE image rows
     if (tid < 256) fs_write_bf(FS_LUT_OFF + 2 * tid, to_bf((float)tid / 255.0f));
+    fs_step_barrier();
     for (int e = tid; e < FS_PRE * FS_IMG_W; e += FS_WAVES * 64) {
         const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
         const int row = t_begin - 1 + rr;
-        fs_write_u8(img_row(row) + cc, fs_load_img(a, row, img_x + cc));
+        const uint8_t u = fs_load_img(a, row, img_x + cc);
+        const uint32_t ix = img_idx(row) + cc;
+        fs_write_u8(FS_IMG_OFF + ix, u);
+        fs_write_bf(FS_IMGB_OFF + 2 * ix, fs_read_bf(FS_LUT_OFF + 2 * u));
     }
     fs_step_barrier();
 
@@ -831,35 +837,27 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                 for (int j = 0; j < 4; j++) {
                     const int gy = r + f_dy[j];
                     rowok[j] = f_tap[j] && gy >= 0 && gy < H;
-                    rb[j] = img_row(gy);
+                    rb[j] = FS_IMGB_OFF + 2 * img_idx(gy);
                 }
-                const int mcount = wave == 4 ? 2 : 1;
-                for (int mi = 0; mi < mcount; mi++) {
-                    const int m = mi == 0 ? wave - 4 : 4;
-                    const int c = 16 * m + i16;
-                    bf16x8 av;
+                const int c = 16 * (wave - 4) + i16;          // M-tile wave - 4 of 64 columns
+                bf16x8 av;
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        const int col = c + 1 + f_dx[j];
-                        const int gx = img_x + col;
-                        const bool ok = rowok[j] && gx >= 0 && gx < W;
-                        const uint8_t u = fs_read_u8(rb[j] + col);
-                        const __bf16 v = fs_read_bf(FS_LUT_OFF + 2 * u);
-                        av[2 * j] = ok ? v : to_bf(0.0f);
-                        av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
-                    }
-                    f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
-                    acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[0], av, acc0, 0, 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[1], av, acc1, 0, 0, 0);
-                    const int gx = x0 - 5 + c;
-                    const bool in = row_in && gx >= 0 && gx < W;
-                    const bool keep = c < fs_w(0);
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++) {
-                        const uint32_t ad = wb + fs_col(c, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
-                        fs_write_b64(keep ? ad : FS_DUMP + 8 * (lane & 7), fs_act4(nt ? acc1 : acc0, xb[nt], in));
-                    }
+                for (int j = 0; j < 4; j++) {
+                    const int col = c + 1 + f_dx[j];
+                    const int gx = img_x + col;
+                    const bool ok = rowok[j] && gx >= 0 && gx < W;
+                    const __bf16 v = fs_read_bf(rb[j] + 2 * col);
+                    av[2 * j] = ok ? v : to_bf(0.0f);
+                    av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
                 }
+                f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[0], av, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[1], av, acc1, 0, 0, 0);
+                const int gx = x0 - 5 + c;
+                const bool in = row_in && gx >= 0 && gx < W;
+#pragma unroll
+                for (int nt = 0; nt < 2; nt++)
+                    fs_write_b64(wb + fs_col(c, 2 * nt + (kq >> 1)) + (kq & 1) * 8, fs_act4(nt ? acc1 : acc0, xb[nt], in));
             }
             if (phase == FS_CHUNK - 1) {
                 const int r0 = t - FS_CHUNK + FS_PRE;            // the rows prefetched at phase 0
@@ -868,7 +866,9 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                     const int e = ptid + 256 * j;
                     if (e < FS_CHUNK_BYTES) {
                         const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-                        fs_write_u8(img_row(r0 + rr) + cc, pf[j]);
+                        const uint32_t ix = img_idx(r0 + rr) + cc;
+                        fs_write_u8(FS_IMG_OFF + ix, pf[j]);
+                        fs_write_bf(FS_IMGB_OFF + 2 * ix, fs_read_bf(FS_LUT_OFF + 2 * pf[j]));
                     }
                 }
             }
@@ -948,7 +948,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                 const int c = 16 * wave + i16;
                 const int gx = x0 + c;
                 if (kq == 0 && c < FS_W && gx < W) {
-                    const uint8_t px = fs_read_u8(img_row(r) + c + 6);
+                    const uint8_t px = fs_read_u8(FS_IMG_OFF + img_idx(r) + c + 6);
                     const float v = (float)px / 255.0f + (acc[0] + xb[0][0]);
                     const float q = rintf(v * 255.0f);
                     a.out[(int64_t)r * a.out_pitch + gx] = (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
