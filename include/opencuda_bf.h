@@ -253,6 +253,43 @@ int bf_synth_image(unsigned char *dst, int32_t width, int32_t height,
                    int32_t nshapes, int32_t smin, int32_t smax,
                    double noise_sigma, uint64_t seed);
 
+/* ------------------------------------------------------------------
+ * The stages around the filter in okano_3_0/SimpleRegionDetect.cu:287-312
+ * (detectRegion: ... -> LocalCluster -> BilateralFilter -> ConnectRegion).
+ * ------------------------------------------------------------------ */
+
+/* LocalCluster's parameters (okano_3_0/LocalCluster.h members gapThred,
+ * diffeThred, proBlack, proWhite, pntRange, pntCount). */
+typedef struct bf_local_cluster_params {
+    int32_t pnt_range;      /* points walked per direction */
+    int32_t pnt_count;      /* directional averages kept (of 8) */
+    uint8_t gap_thred;      /* max |centre - walked pixel| */
+    uint8_t diffe_thred;    /* max difference of the two side sums */
+    uint8_t pro_black;      /* pixels <= pro_black are copied */
+    uint8_t pro_white;      /* pixels >= pro_white are copied */
+} bf_local_cluster_params;
+
+/* Replaces LocalCluster::localCluster's kernel launch
+ * (okano_3_0/LocalCluster.cu:45-153, launch :246-270).  `in` is the ROI
+ * view; [alloc_lo, alloc_hi) are the byte offsets from in->data that may be
+ * read (the whole allocation holding the ROI): the reference's walks read
+ * up to 2 (pnt_range - 1) pixels from a pixel by linear address and reads
+ * outside the allocation are clamped into it.  alloc_lo = alloc_hi = 0
+ * means the ROI's own bytes.  Output size = min of the two views.  Async on
+ * `stream`. */
+int bf_local_cluster(const bf_image_view *in, int64_t alloc_lo, int64_t alloc_hi,
+                     const bf_image_view *out, const bf_local_cluster_params *params,
+                     void *stream);
+
+/* Replaces ConnectRegion::connectRegion (okano_3_0/ConnectRegion.cu:789-1043,
+ * which the reference calls with min_gray 10, max_gray 250): 8-connected
+ * regions of pixels in [min_gray, max_gray], area filter [min_area,
+ * max_area], regions numbered in raster order of their first pixel, u8
+ * output = number + 1 (0 = none).  Output size = min of the two views.
+ * Allocates 12 bytes per pixel of stream-ordered scratch.  Async. */
+int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t min_area,
+                      int32_t max_area, int32_t min_gray, int32_t max_gray, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,13 @@ class View(ctypes.Structure):
                 ("height", ctypes.c_int32), ("pitch", ctypes.c_int64)]
 
 
+class LocalClusterParams(ctypes.Structure):
+    """bf_local_cluster_params (LocalCluster's members)."""
+    _fields_ = [("pnt_range", ctypes.c_int32), ("pnt_count", ctypes.c_int32),
+                ("gap_thred", ctypes.c_uint8), ("diffe_thred", ctypes.c_uint8),
+                ("pro_black", ctypes.c_uint8), ("pro_white", ctypes.c_uint8)]
+
+
 class Job(ctypes.Structure):
     """bf_job: one image of a batch."""
     _fields_ = [("inp", View), ("out", View), ("lut", ctypes.c_void_p),
@@ -98,6 +105,9 @@ def lib() -> ctypes.CDLL:
         "bf_version": ([], ctypes.c_int),
         "bf_synth_image": ([P, I32, I32, I32, I32, I32, ctypes.c_double, ctypes.c_uint64],
                            ctypes.c_int),
+        "bf_local_cluster": ([VP, I64, I64, VP, ctypes.POINTER(LocalClusterParams), P],
+                             ctypes.c_int),
+        "bf_connect_region": ([VP, VP, I32, I32, I32, I32, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -256,6 +266,37 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------------- region stages
+
+def local_cluster(vin: View, vout: View, pnt_range: int = 8, pnt_count: int = 5,
+                  gap_thred: int = 10, diffe_thred: int = 10, pro_black: int = 10,
+                  pro_white: int = 250, alloc=(0, 0), stream=None) -> None:
+    """bf_local_cluster: LocalCluster (okano_3_0/LocalCluster.cu) on a ROI
+    view; `alloc` = readable byte range [lo, hi) around vin.data (see
+    opencuda_bf.h), (0, 0) = the ROI itself.  Defaults are the
+    parameterised constructor's."""
+    p = LocalClusterParams(int(pnt_range), int(pnt_count), int(gap_thred), int(diffe_thred),
+                           int(pro_black), int(pro_white))
+    _check(lib().bf_local_cluster(ctypes.byref(vin), int(alloc[0]), int(alloc[1]),
+                                  ctypes.byref(vout), ctypes.byref(p), _stream(stream)),
+           "bf_local_cluster")
+
+
+def alloc_range(t, roi) -> tuple:
+    """[lo, hi) byte offsets of tensor t's storage around the ROI origin."""
+    x1, y1 = roi[0], roi[1]
+    origin = y1 * t.stride(0) + x1
+    return -origin, t.shape[0] * t.stride(0) - origin
+
+
+def connect_region(vin: View, vout: View, min_area: int = 100, max_area: int = 100000,
+                   min_gray: int = 10, max_gray: int = 250, stream=None) -> None:
+    """bf_connect_region: ConnectRegion (okano_3_0/ConnectRegion.cu)."""
+    _check(lib().bf_connect_region(ctypes.byref(vin), ctypes.byref(vout), int(min_area),
+                                   int(max_area), int(min_gray), int(max_gray), _stream(stream)),
+           "bf_connect_region")
 
 
 # ------------------------------------------------------------ synthetic data

@@ -1,0 +1,425 @@
+// region.hip -- the stages around the bilateral filter in
+// okano_3_0/SimpleRegionDetect.cu:287-312: LocalCluster before it and
+// ConnectRegion after it, for gfx950.
+//
+// LocalCluster (okano_3_0/LocalCluster.cu:45-153): one lane per pixel runs
+// the eight directional walks the reference spreads over eight threads and
+// a shared array, with the same FP32 operations in the same order
+// (-ffp-contract=off, IEEE division), so the u8 result is the reference's.
+// The reference addresses every read as data + y * pitch + x from the ROI
+// origin and its walks reach 2 (pntRange - 1) pixels from the centre while
+// the guard band is pntRange + 2 wide; reads here use the same linear
+// address, clamped into the allocation the caller declares.
+//
+// ConnectRegion (okano_3_0/ConnectRegion.cu:789-1043): 8-connected
+// components of pixels whose gray values lie in [minGray, maxGray], each
+// labelled by its minimum raster index.  The reference gets there with
+// shared-memory labelling per block and rounds of border merging; here one
+// lock-free union-find pass (atomicMin on roots, parents always smaller
+// than children, so every loop ends) over each pixel's four earlier
+// neighbours, then path compression -- the result does not depend on the
+// order the unions run in.  Areas, the area filter, the raster-order
+// renumbering and the u8 output follow the reference, including its area
+// launch covering only rows below 256 * ceil(floor((H + 4) / 32) / 8)
+// (ConnectRegion.cu:45, :957-967) and never dropping a root whose counted
+// area is 0.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+
+#include "bf_internal.h"
+
+namespace bfimpl {
+namespace region {
+
+// ------------------------------------------------------------ LocalCluster
+
+struct LcArgs {
+    const uint8_t *in;        // ROI origin
+    int64_t pitch;
+    int64_t lo, hi;           // readable byte offsets from `in`: [lo, hi)
+    uint8_t *out;
+    int64_t out_pitch;
+    int W, H;
+    int pnt_range, pnt_count;
+    float gap, diffe;
+    int pro_black, pro_white;
+};
+
+__device__ __forceinline__ uint32_t lc_read(const LcArgs &a, int y, int x)
+{
+    int64_t off = (int64_t)y * a.pitch + x;
+    off = off < a.lo ? a.lo : (off >= a.hi ? a.hi - 1 : off);
+    return a.in[off];
+}
+
+__device__ __forceinline__ uint8_t lc_sat_u8(float q)
+{
+    if (!(q > 0.0f)) return 0;
+    if (q >= 255.0f) return 255;
+    return (uint8_t)(int)q;
+}
+
+__global__ void __launch_bounds__(256) local_cluster_kernel(LcArgs a)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.W || y >= a.H) return;
+    const uint32_t cv = lc_read(a, y, x);
+    const int sx = a.pnt_range + 2, ex = a.W - a.pnt_range - 2;
+    const int sy = a.pnt_range + 2, ey = a.H - a.pnt_range - 2;
+    uint8_t *dst = a.out + (int64_t)y * a.out_pitch + x;
+    if (x < sx || x > ex || y < sy || y > ey || (int)cv <= a.pro_black || (int)cv >= a.pro_white) {
+        *dst = (uint8_t)cv;
+        return;
+    }
+    // directions {dx, dy} of LocalCluster.cu:14-19
+    const int ddx_t[8] = { 1, 1, 0, -1, -1, -1, 0, 1 };
+    const int ddy_t[8] = { 0, -1, -1, -1, 0, 1, 1, 1 };
+    const float fcv = (float)cv;
+    float temp[8];
+#pragma unroll
+    for (int z = 0; z < 8; z++) {
+        const int ddx = ddx_t[z], ddy = ddy_t[z];
+        uint32_t cuv = lc_read(a, y + ddy, x + ddx);
+        uint32_t pre1 = lc_read(a, y - ddy, x - ddx);
+        uint32_t pre2 = cv;
+        uint32_t latt1 = lc_read(a, y + 2 * ddy, x + 2 * ddx);
+        float sum = fcv;
+        int i, px = x, py = y;
+        for (i = 1; i < a.pnt_range; i++) {
+            if (fabsf(fcv - (float)cuv) > a.gap) break;
+            px += 2 * ddx;
+            py += 2 * ddy;
+            const uint32_t latt2 = lc_read(a, py, px);
+            const float side1 = (float)(pre1 + pre2);
+            const float side2 = (float)(latt1 + latt2);
+            if (fabsf(side1 - side2) > a.diffe) break;
+            pre1 = pre2;
+            pre2 = cuv;
+            cuv = latt1;
+            latt1 = latt2;
+            sum += (float)cuv;
+        }
+        temp[z] = sum / (float)i;
+    }
+    int mark = 0;
+    for (int j = 8 - a.pnt_count; j > 0; j--) {
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            if (fabsf(temp[i] - fcv) >= fabsf(temp[mark] - fcv)) mark = i;
+#pragma unroll
+        for (int i = 0; i < 8; i++) temp[i] = i == mark ? fcv : temp[i];
+    }
+    float sumag = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; j++) sumag += temp[j];
+    sumag -= (float)((8 - a.pnt_count) * (int)cv);
+    const float q = sumag / (float)a.pnt_count;
+    *dst = q > 255.0f ? 255 : lc_sat_u8(q);
+}
+
+// ----------------------------------------------------------- ConnectRegion
+
+__device__ __forceinline__ int cr_load(const int *L, int i)
+{
+    return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Root of x: parents are never larger than their children, so the walk ends.
+__device__ __forceinline__ int cr_find(const int *L, int x)
+{
+    int p = cr_load(L, x);
+    while (p != x) {
+        x = p;
+        p = cr_load(L, x);
+    }
+    return x;
+}
+
+// Lock-free union (the larger root is hooked under the smaller one with
+// atomicMin; a failed hook means that root was hooked meanwhile, and the
+// loop continues from its new parent, a strictly smaller index).
+__device__ void cr_unite(int *L, int a, int b)
+{
+    while (true) {
+        a = cr_find(L, a);
+        b = cr_find(L, b);
+        if (a == b) return;
+        if (a < b) {
+            const int old = atomicMin(L + b, a);
+            if (old == b) return;
+            b = old;
+        } else {
+            const int old = atomicMin(L + a, b);
+            if (old == a) return;
+            a = old;
+        }
+    }
+}
+
+struct CrArgs {
+    const uint8_t *in;
+    int64_t pitch;
+    uint8_t *out;
+    int64_t out_pitch;
+    int W, H;
+    int min_gray, max_gray;
+    int min_area, max_area;
+    int covered;              // rows the area count covers
+    int *label, *area, *seq, *rowcount, *rowoff;
+};
+
+__device__ __forceinline__ bool cr_fg(const CrArgs &a, uint32_t v)
+{
+    return (int)v >= a.min_gray && (int)v <= a.max_gray;
+}
+
+__global__ void cr_init_kernel(CrArgs a)
+{
+    const int64_t n = (int64_t)a.W * a.H;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        a.label[i] = (int)i;
+        a.area[i] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) cr_union_kernel(CrArgs a)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.W || y >= a.H) return;
+    if (!cr_fg(a, a.in[(int64_t)y * a.pitch + x])) return;
+    const int idx = y * a.W + x;
+    if (x > 0 && cr_fg(a, a.in[(int64_t)y * a.pitch + x - 1])) cr_unite(a.label, idx, idx - 1);
+    if (y > 0) {
+        const uint8_t *up = a.in + (int64_t)(y - 1) * a.pitch;
+        if (x > 0 && cr_fg(a, up[x - 1])) cr_unite(a.label, idx, idx - a.W - 1);
+        if (cr_fg(a, up[x])) cr_unite(a.label, idx, idx - a.W);
+        if (x + 1 < a.W && cr_fg(a, up[x + 1])) cr_unite(a.label, idx, idx - a.W + 1);
+    }
+}
+
+__global__ void cr_compress_kernel(CrArgs a)
+{
+    const int64_t n = (int64_t)a.W * a.H;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a.label[i] = cr_find(a.label, (int)i);
+}
+
+// Areas per root over runs of 32 rows of one column (fewer atomics on the
+// large components), rows below a.covered only.
+__global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
+{
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    const int y0 = blockIdx.y * 32;
+    if (x >= a.W || y0 >= a.covered) return;
+    const int y1 = min(y0 + 32, a.covered);
+    int cur = a.label[y0 * a.W + x], run = 1;
+    for (int y = y0 + 1; y < y1; y++) {
+        const int l = a.label[y * a.W + x];
+        if (l != cur) {
+            atomicAdd(a.area + cur, run);
+            cur = l;
+            run = 1;
+        } else {
+            run++;
+        }
+    }
+    atomicAdd(a.area + cur, run);
+}
+
+__device__ __forceinline__ bool cr_kept_root(const CrArgs &a, int i)
+{
+    if (a.label[i] != i) return false;
+    const int ar = a.area[i];
+    return !(ar != 0 && (ar < a.min_area || ar > a.max_area));
+}
+
+// One wave per row: kept roots in the row.
+__global__ void __launch_bounds__(64) cr_rowcount_kernel(CrArgs a)
+{
+    const int y = blockIdx.x;
+    int count = 0;
+    for (int x0 = 0; x0 < a.W; x0 += 64) {
+        const int x = x0 + (int)threadIdx.x;
+        const bool k = x < a.W && cr_kept_root(a, y * a.W + x);
+        count += __popcll(__ballot(k));
+    }
+    if (threadIdx.x == 0) a.rowcount[y] = count;
+}
+
+// Exclusive scan of the row counts (one workgroup).
+__global__ void __launch_bounds__(1024) cr_scan_kernel(CrArgs a)
+{
+    __shared__ int part[1024];
+    __shared__ int carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < a.H; base += 1024) {
+        const int i = base + (int)threadIdx.x;
+        const int v = i < a.H ? a.rowcount[i] : 0;
+        part[threadIdx.x] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            const int t = threadIdx.x >= (unsigned)off ? part[threadIdx.x - off] : 0;
+            __syncthreads();
+            part[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < a.H) a.rowoff[i] = carry + part[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += part[1023];
+        __syncthreads();
+    }
+}
+
+// One wave per row: raster-order numbers of the kept roots (-1 for the
+// other roots and for numbers >= 256).
+__global__ void __launch_bounds__(64) cr_number_kernel(CrArgs a)
+{
+    const int y = blockIdx.x;
+    const uint64_t below = (1ull << threadIdx.x) - 1ull;
+    int next = a.rowoff[y];
+    for (int x0 = 0; x0 < a.W; x0 += 64) {
+        const int x = x0 + (int)threadIdx.x;
+        const int i = y * a.W + x;
+        const bool in = x < a.W;
+        const bool root = in && a.label[i] == i;
+        const bool k = in && cr_kept_root(a, i);
+        const uint64_t m = __ballot(k);
+        if (root) {
+            const int s = next + __popcll(m & below);
+            a.seq[i] = (k && s < 256) ? s : -1;
+        }
+        next += __popcll(m);
+    }
+}
+
+__global__ void __launch_bounds__(256) cr_output_kernel(CrArgs a)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.W || y >= a.H) return;
+    const int s = a.seq[a.label[y * a.W + x]];
+    a.out[(int64_t)y * a.out_pitch + x] = (uint8_t)(s + 1);
+}
+
+}  // namespace region
+}  // namespace bfimpl
+
+using namespace bfimpl;
+using namespace bfimpl::region;
+
+namespace {
+
+bool view_ok(const bf_image_view *v)
+{
+    return v && v->data && v->width > 0 && v->height > 0 && v->pitch >= v->width;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bf_local_cluster(const bf_image_view *in, int64_t alloc_lo, int64_t alloc_hi,
+                     const bf_image_view *out, const bf_local_cluster_params *p, void *stream)
+{
+    if (!in || !out || !p || !in->data || !out->data) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    if (!view_ok(in) || !view_ok(out)) {
+        set_error("empty or malformed view");
+        return kInvalidData;
+    }
+    if (alloc_lo == 0 && alloc_hi == 0) {          // the ROI itself
+        alloc_hi = (int64_t)(in->height - 1) * in->pitch + in->width;
+    }
+    if (alloc_lo > 0 || alloc_hi < (int64_t)(in->height - 1) * in->pitch + in->width) {
+        set_error("declared allocation does not contain the ROI");
+        return kInvalidData;
+    }
+    LcArgs a{};
+    a.in = in->data;
+    a.pitch = in->pitch;
+    a.lo = alloc_lo;
+    a.hi = alloc_hi;
+    a.out = out->data;
+    a.out_pitch = out->pitch;
+    a.W = std::min(in->width, out->width);
+    a.H = std::min(in->height, out->height);
+    a.pnt_range = p->pnt_range;
+    a.pnt_count = p->pnt_count;
+    a.gap = (float)p->gap_thred;
+    a.diffe = (float)p->diffe_thred;
+    a.pro_black = p->pro_black;
+    a.pro_white = p->pro_white;
+    hipLaunchKernelGGL(local_cluster_kernel, dim3((a.W + 63) / 64, (a.H + 3) / 4), dim3(256), 0,
+                       (hipStream_t)stream, a);
+    BF_HIP(hipGetLastError());
+    return kOk;
+}
+
+int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t min_area,
+                      int32_t max_area, int32_t min_gray, int32_t max_gray, void *stream)
+{
+    if (!in || !out || !in->data || !out->data) {
+        set_error("null argument");
+        return kNullPointer;
+    }
+    if (!view_ok(in) || !view_ok(out)) {
+        set_error("empty or malformed view");
+        return kInvalidData;
+    }
+    const int W = std::min(in->width, out->width), H = std::min(in->height, out->height);
+    if ((int64_t)W * H > 0x7FFFFFFF) {
+        set_error("image too large for 32-bit labels");
+        return kInvalidData;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    const size_t n = (size_t)W * H;
+    const size_t bytes = (3 * n + 2 * (size_t)H) * sizeof(int);
+    int *work = nullptr;
+    BF_HIP(hipMallocAsync(reinterpret_cast<void **>(&work), bytes, s));
+    CrArgs a{};
+    a.in = in->data;
+    a.pitch = in->pitch;
+    a.out = out->data;
+    a.out_pitch = out->pitch;
+    a.W = W;
+    a.H = H;
+    a.min_gray = min_gray;
+    a.max_gray = max_gray;
+    a.min_area = min_area;
+    a.max_area = max_area;
+    a.covered = std::min(H, 256 * ((((H + 4) / 32) + 7) / 8));
+    a.label = work;
+    a.area = work + n;
+    a.seq = work + 2 * n;
+    a.rowcount = work + 3 * n;
+    a.rowoff = work + 3 * n + H;
+    const int flat = (int)std::min<size_t>((n + 255) / 256, 65536);
+    const dim3 g2((W + 63) / 64, (H + 3) / 4);
+    hipLaunchKernelGGL(cr_init_kernel, dim3(flat), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(cr_union_kernel, g2, dim3(256), 0, s, a);
+    hipLaunchKernelGGL(cr_compress_kernel, dim3(flat), dim3(256), 0, s, a);
+    if (a.covered > 0)
+        hipLaunchKernelGGL(cr_area_kernel, dim3((W + 255) / 256, (a.covered + 31) / 32), dim3(256), 0,
+                           s, a);
+    hipLaunchKernelGGL(cr_rowcount_kernel, dim3(H), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(cr_scan_kernel, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(cr_number_kernel, dim3(H), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(cr_output_kernel, g2, dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    hipError_t f = hipFreeAsync(work, s);
+    if (e != hipSuccess) return hip_fail(e, "bf_connect_region");
+    if (f != hipSuccess) return hip_fail(f, "bf_connect_region");
+    return kOk;
+}
+
+}  // extern "C"

@@ -236,6 +236,8 @@ def _main() -> int:
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     ap.add_argument("--cosine", action="store_true", help="decay lr along a half cosine")
     ap.add_argument("--eval", action="store_true", help="report accuracy on the c3 image")
+    ap.add_argument("--init", default=None, help="start from these weights (.npz)")
+    ap.add_argument("--log-every", type=int, default=100)
     args = ap.parse_args()
 
     def log(m):
@@ -244,11 +246,16 @@ def _main() -> int:
     mode = MODE_STD if args.mode == "std" else MODE_REF
     data = Dataset(args.images, args.size, mode=mode, log=log)
     net = Surrogate(args.seed)
+    init_steps = 0
+    if args.init:
+        init_steps = int(net.load(args.init).get("steps", 0))
+        log(f"starting from {args.init} ({init_steps} steps)")
     hist = train(net, data, args.steps, args.batch, args.crop, args.lr, log=log,
-                 cosine=args.cosine)
+                 cosine=args.cosine, log_every=args.log_every)
     net.save(args.out, history=np.array(hist, np.float64), mode=mode, radius=15,
-             sigma_space=10.0, images=args.images, steps=args.steps, batch=args.batch,
-             crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine))
+             sigma_space=10.0, images=args.images, steps=init_steps + args.steps, batch=args.batch,
+             crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine),
+             init_steps=init_steps)
     log(f"saved {args.out}")
     if args.eval:
         from . import workloads

@@ -80,6 +80,13 @@ def lib() -> ctypes.CDLL:
         L.orc_filter_f64.restype = ctypes.c_int
         L.orc_set_mutation.argtypes = [ctypes.c_int]
         L.orc_set_mutation.restype = None
+        I64 = ctypes.c_int64
+        L.orc_local_cluster.argtypes = [P, I64, I64, I64, ctypes.c_int, ctypes.c_int, P, I64] + \
+            [ctypes.c_int] * 6
+        L.orc_local_cluster.restype = ctypes.c_int
+        L.orc_connect_region.argtypes = [P, I64, ctypes.c_int, ctypes.c_int, P, I64] + \
+            [ctypes.c_int] * 4
+        L.orc_connect_region.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -181,4 +188,42 @@ def filter_f64(img: np.ndarray, radius: int, sigma_space: float, sigma_range: fl
                               float(sigma_range), int(mode), int(nthreads), int(y0), int(y1))
     if rc != 0:
         raise ValueError(f"orc_filter_f64 failed: {rc}")
+    return out
+
+
+# ------------------------------------------------------------ region stages
+
+def local_cluster(img: np.ndarray, roi=None, out_size=None, pntrange: int = 8,
+                  gapthred: int = 10, diffethred: int = 10, problack: int = 10,
+                  prowhite: int = 250, pntcount: int = 5) -> np.ndarray:
+    """okano_3_0/LocalCluster.cu restated (region_oracle.c).  The image array
+    is the whole allocation; reads the reference would make outside it are
+    clamped into it.  Defaults are the parameterised constructor's
+    (LocalCluster.h)."""
+    base, off, pitch, w, h = _view(img, roi)
+    if out_size is not None:
+        w, h = min(w, out_size[0]), min(h, out_size[1])
+    out = np.zeros((h, w), np.uint8)
+    lo, hi = -off, img.size - off
+    flat = base.reshape(-1)
+    rc = lib().orc_local_cluster(flat.ctypes.data + off, pitch, lo, hi, w, h, _ptr(out), w,
+                                 pntrange, gapthred, diffethred, problack, prowhite, pntcount)
+    if rc != 0:
+        raise ValueError(f"orc_local_cluster: {rc}")
+    return out
+
+
+def connect_region(img: np.ndarray, roi=None, out_size=None, min_area: int = 100,
+                   max_area: int = 100000, mingray: int = 10, maxgray: int = 250) -> np.ndarray:
+    """okano_3_0/ConnectRegion.cu restated (region_oracle.c): u8 region
+    numbers (1..255, 0 = no region)."""
+    base, off, pitch, w, h = _view(img, roi)
+    if out_size is not None:
+        w, h = min(w, out_size[0]), min(h, out_size[1])
+    out = np.zeros((h, w), np.uint8)
+    flat = base.reshape(-1)
+    rc = lib().orc_connect_region(flat.ctypes.data + off, pitch, w, h, _ptr(out), w,
+                                  min_area, max_area, mingray, maxgray)
+    if rc != 0:
+        raise ValueError(f"orc_connect_region: {rc}")
     return out
