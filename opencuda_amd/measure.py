@@ -3,6 +3,7 @@
     python -m opencuda_amd.measure c2 [--steps 5]     # batched persistent launch
     python -m opencuda_amd.measure c4 [--steps 3]     # exact batch + surrogate, one graph
     python -m opencuda_amd.measure c3-surrogate       # the surrogate on the 8192^2 image
+    python -m opencuda_amd.measure chain              # LocalCluster -> filter -> ConnectRegion
 
 c2 (256 images, 256..2048 per side, ROI 50..100 %, r in {3,5,9}, REF mode):
 one bf_batch launch (eager and replayed from a captured graph) against the
@@ -14,6 +15,10 @@ against one bf_batch launch that filters every image exactly.
 
 c3-surrogate (8192^2, r=15, STD target): fused BF16 inference alone, then
 inference + sampled exact check + exact fallback of failing 64x64 tiles.
+
+chain (4096^2 synthetic image): LocalCluster (pntRange 8, pntCount 5,
+thresholds 10), the exact filter (REF, r = 7, sigma_space 5, sigma_range
+25), ConnectRegion (areas 100..100000), each stage timed on its own.
 
 Each prints one JSON line; progress goes to stderr.  Inputs are resident in
 HBM before timing starts; times come from HIP events on the launch stream.
@@ -166,9 +171,41 @@ def c3_surrogate(steps: int = 5):
     }
 
 
+def chain(steps: int = 5, size: int = 4096):
+    import torch
+    from . import alloc_range, connect_region, local_cluster
+    img = wl.Item(size, size, 400, 16, 512, 10.0, wl.sm64(5), (0, 0, size, size), 7, 5.0,
+                  25.0).image()
+    tin = torch.from_numpy(img).cuda()
+    lc, bf, cr = (torch.zeros_like(tin) for _ in range(3))
+    lut = Lut.from_sigmas(5.0, 25.0)
+    stream = torch.cuda.current_stream()
+    full = (0, 0, size, size)
+    stages = {
+        "local_cluster": lambda: local_cluster(view(tin), view(lc), alloc=alloc_range(tin, full),
+                                               stream=stream),
+        "bilateral_filter": lambda: exact(view(lc), view(bf), lut, 7, MODE_REF, stream=stream),
+        "connect_region": lambda: connect_region(view(bf), view(cr), 100, 100000, stream=stream),
+    }
+    for f in stages.values():
+        f()
+    torch.cuda.synchronize()
+    px = size * size
+    res = {"workload": f"chain on a {size}x{size} synthetic image", "pixels": px}
+    total = 0.0
+    for name, f in stages.items():
+        ms = _time(f, steps, stream)
+        total += ms
+        res[name] = {"ms": round(ms, 4), "mpix_s": round(px / ms / 1e3, 1)}
+        _log(f"{name}: {ms:.3f} ms")
+    res["chain_ms"] = round(total, 4)
+    res["regions"] = int(cr.max().item())
+    return res
+
+
 def _main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["c2", "c4", "c3-surrogate"])
+    ap.add_argument("workload", choices=["c2", "c4", "c3-surrogate", "chain"])
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--count", type=int, default=512, help="c4: number of images")
     ap.add_argument("--mode", choices=["ref", "std"], default=None)
@@ -178,6 +215,8 @@ def _main() -> int:
     torch.cuda.set_device(0)
     if args.workload == "c3-surrogate":
         res = c3_surrogate(args.steps or 5)
+    elif args.workload == "chain":
+        res = chain(args.steps or 5)
     elif args.workload == "c2":
         mode = MODE_STD if args.mode == "std" else MODE_REF
         res = c2(args.steps or 5, mode, args.tune)

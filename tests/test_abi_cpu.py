@@ -28,6 +28,8 @@ def test_library_loads_and_exports_every_declared_function():
 
 
 CXX_SYMBOLS = [
+    "_ZN12LocalCluster12localClusterEP8Image_stS1_",
+    "_ZN13ConnectRegion13connectRegionEP8Image_stS1_",
     "_ZN12ImageBasicOp8newImageEPP8Image_st",
     "_ZN12ImageBasicOp11deleteImageEP8Image_st",
     "_ZN12ImageBasicOp19makeAtCurrentDeviceEP8Image_stmm",
@@ -76,6 +78,16 @@ def test_argument_validation_without_gpu():
     E = np.zeros(256, np.float32)
     assert L.bf_euclid_template(0.0, 256, E.ctypes.data) == oc.INVALID_DATA
     assert L.bf_euclid_template(10.0, 256, E.ctypes.data) == 256
+    # the region stages reject null and malformed views before any device call
+    p = oc.LocalClusterParams(8, 5, 10, 10, 10, 250)
+    assert L.bf_local_cluster(None, 0, 0, None, ctypes.byref(p), None) == oc.NULL_POINTER
+    assert L.bf_connect_region(None, None, 1, 10, 10, 250, None) == oc.NULL_POINTER
+    bad = oc.View(1234, 8, 8, 4)           # pitch < width
+    assert L.bf_connect_region(ctypes.byref(bad), ctypes.byref(bad), 1, 10, 10, 250,
+                               None) == oc.INVALID_DATA
+    good = oc.View(1234, 8, 8, 8)
+    assert L.bf_local_cluster(ctypes.byref(good), 8, 100, ctypes.byref(good), ctypes.byref(p),
+                              None) == oc.INVALID_DATA    # allocation must contain the ROI
     assert oc.last_error() is not None
 
 
@@ -128,8 +140,10 @@ def test_public_headers_compile_with_plain_gxx(tmp_path):
     """The C++ object model headers need no HIP toolchain."""
     import subprocess
     src = tmp_path / "t.cpp"
-    src.write_text('#include "BilateralFilter.h"\n#include "opencuda_bf.h"\n'
-                   'int main(){ BilateralFilter f; return f.getRadius(); }\n')
+    src.write_text('#include "BilateralFilter.h"\n#include "LocalCluster.h"\n'
+                   '#include "ConnectRegion.h"\n#include "opencuda_bf.h"\n'
+                   'int main(){ BilateralFilter f; LocalCluster l; ConnectRegion c(0, 500, 20);\n'
+                   '  return f.getRadius() + l.getPntRange() + c.getMinArea(); }\n')
     r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
