@@ -22,6 +22,11 @@ surrogate: the BF16 MFMA surrogate with the sampled exact check and exact
 fallback of failing 64x64 tiles (tolerance 2 LSB), against the exact filter
 in STD mode -- the mode the network is trained on -- with error statistics
 over all pixels.
+
+chain: the stages around the filter in the reference's region detector
+(LocalCluster -> exact filter r=7 -> ConnectRegion) on a 4096^2 image, each
+stage timed on the GPU; the CPU leg times the oracle's three stages on a
+1024^2 image.
 """
 import argparse
 import json
@@ -79,6 +84,34 @@ def cpu_baseline(img, it, gpu_out, seconds: float):
         "nproc": os.cpu_count(),
     }
     return base, match, (y0, y0 + rows)
+
+
+def chain_cpu(size: int = 1024):
+    """The oracle's LocalCluster, filter (REF, r = 7) and ConnectRegion on a
+    size^2 image with the chain leg's parameters: Mpix/s per stage."""
+    import numpy as np
+    import oracle
+    from opencuda_amd import workloads as wl
+    img = wl.Item(size, size, 40, 8, 128, 10.0, wl.sm64(5), (0, 0, size, size), 7, 5.0,
+                  25.0).image()
+    nthreads = min(os.cpu_count() or 1, 16)
+    out = {}
+    t = time.perf_counter()
+    lc = oracle.local_cluster(img)
+    out["local_cluster"] = time.perf_counter() - t
+    t = time.perf_counter()
+    bf = oracle.filter(lc, 7, 5.0, 25.0, mode=oracle.REF, nthreads=nthreads)
+    out["bilateral_filter"] = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.connect_region(bf, min_area=100, max_area=100000)
+    out["connect_region"] = time.perf_counter() - t
+    px = img.size
+    res = {k: {"mpix_s": round(px / v / 1e6, 3)} for k, v in out.items()}
+    res["bilateral_filter"]["cores"] = nthreads
+    res["local_cluster"]["cores"] = 1
+    res["connect_region"]["cores"] = 1
+    res["sample"] = f"oracle stages on a {size}x{size} synthetic image"
+    return res
 
 
 def fp64_error(img, it, gpu_out, rows):
@@ -163,6 +196,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate leg")
+    ap.add_argument("--no-chain", action="store_true", help="skip the region-chain leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain 256 MiB device-to-device copy (torch copy_): "
                          "the known-size kernel that calibrates FETCH_SIZE/WRITE_SIZE counters")
@@ -287,12 +321,19 @@ def main() -> int:
         },
     }
     result["surrogate"] = sur
+    if not args.no_chain:
+        from opencuda_amd import measure
+        log("chain leg (LocalCluster -> filter -> ConnectRegion, 4096^2)")
+        result["chain"] = measure.chain(steps=max(3, args.steps // 4))
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         base, match, rows = cpu_baseline(img, it, gpu_out, args.cpu_seconds)
         result["cpu_baseline"] = base
         result["bit_exact_sample"] = match
         result["fp64_error"] = fp64_error(img, it, gpu_out, (rows[0], rows[0] + 16))
+        if not args.no_chain:
+            log("cpu chain stages (oracle, 1024^2)")
+            result["cpu_baseline"]["chain"] = chain_cpu()
     else:
         result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)

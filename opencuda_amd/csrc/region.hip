@@ -139,14 +139,32 @@ __device__ __forceinline__ int cr_find(const int *L, int x)
     return x;
 }
 
+// Root of x with path halving: each visited node is re-pointed to its
+// grandparent, an ancestor it keeps (trees only ever merge), so parents stay
+// smaller than children.  A hook another lane makes meanwhile on a node
+// this walk re-points could be overwritten, but such a node was no longer a
+// root, so that lane's union loop does not stop there and merges the trees
+// through the node's former parent.
+__device__ __forceinline__ int cr_find_halving(int *L, int x)
+{
+    while (true) {
+        const int p = cr_load(L, x);
+        if (p == x) return x;
+        const int gp = cr_load(L, p);
+        if (gp == p) return p;
+        __hip_atomic_store(L + x, gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+    }
+}
+
 // Lock-free union (the larger root is hooked under the smaller one with
 // atomicMin; a failed hook means that root was hooked meanwhile, and the
 // loop continues from its new parent, a strictly smaller index).
 __device__ void cr_unite(int *L, int a, int b)
 {
     while (true) {
-        a = cr_find(L, a);
-        b = cr_find(L, b);
+        a = cr_find_halving(L, a);
+        b = cr_find_halving(L, b);
         if (a == b) return;
         if (a < b) {
             const int old = atomicMin(L + b, a);
@@ -211,26 +229,48 @@ __global__ void cr_compress_kernel(CrArgs a)
         a.label[i] = cr_find(a.label, (int)i);
 }
 
-// Areas per root over runs of 32 rows of one column (fewer atomics on the
-// large components), rows below a.covered only.
+// Areas per root over runs of up to 64 rows of one column, rows below
+// a.covered only.  A run that ends at a label change is added by its lane;
+// the runs that reach the segment's end are combined across the wave first
+// (one atomic per distinct label), since inside a large region every lane
+// ends on the same root.
+constexpr int kAreaRows = 64;
+
 __global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
 {
     const int x = blockIdx.x * 256 + threadIdx.x;
-    const int y0 = blockIdx.y * 32;
-    if (x >= a.W || y0 >= a.covered) return;
-    const int y1 = min(y0 + 32, a.covered);
-    int cur = a.label[y0 * a.W + x], run = 1;
-    for (int y = y0 + 1; y < y1; y++) {
-        const int l = a.label[y * a.W + x];
-        if (l != cur) {
-            atomicAdd(a.area + cur, run);
-            cur = l;
-            run = 1;
-        } else {
-            run++;
+    const int y0 = blockIdx.y * kAreaRows;
+    const bool active = x < a.W && y0 < a.covered;
+    int cur = -1, run = 0;
+    if (active) {
+        const int y1 = min(y0 + kAreaRows, a.covered);
+        cur = a.label[y0 * a.W + x];
+        run = 1;
+        for (int y = y0 + 1; y < y1; y++) {
+            const int l = a.label[y * a.W + x];
+            if (l != cur) {
+                atomicAdd(a.area + cur, run);
+                cur = l;
+                run = 1;
+            } else {
+                run++;
+            }
         }
     }
-    atomicAdd(a.area + cur, run);
+    // combine the final runs of the wave, one distinct label at a time
+    const int lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(active);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int lab = __shfl(cur, leader);
+        const bool mine = active && cur == lab;
+        const uint64_t group = __ballot(mine);
+        int sum = mine ? run : 0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if (lane == leader) atomicAdd(a.area + lab, sum);
+        todo &= ~group;
+    }
 }
 
 __device__ __forceinline__ bool cr_kept_root(const CrArgs &a, int i)
@@ -409,8 +449,8 @@ int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t
     hipLaunchKernelGGL(cr_union_kernel, g2, dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_compress_kernel, dim3(flat), dim3(256), 0, s, a);
     if (a.covered > 0)
-        hipLaunchKernelGGL(cr_area_kernel, dim3((W + 255) / 256, (a.covered + 31) / 32), dim3(256), 0,
-                           s, a);
+        hipLaunchKernelGGL(cr_area_kernel, dim3((W + 255) / 256, (a.covered + kAreaRows - 1) / kAreaRows),
+                           dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_rowcount_kernel, dim3(H), dim3(64), 0, s, a);
     hipLaunchKernelGGL(cr_scan_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(cr_number_kernel, dim3(H), dim3(64), 0, s, a);

@@ -609,8 +609,8 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
 // pixel and stores them with one 8-byte write.
 //
 // Activations are 64-byte NHWC pixels in ring rows whose pitch is a
-// multiple of 16 pixels; the 16-byte chunk c of the pixel in column x sits
-// at chunk c ^ ((x >> 2) & 3), so the 16 consecutive pixels one
+// multiple of 16 pixels; the 16-byte chunks of a pixel are permuted by a
+// function of its column mod 16 (fs_col), so the consecutive pixels one
 // ds_read_b128 gathers fall into distinct banks, and every per-lane part of
 // an address is the same at every step.
 //
@@ -682,9 +682,14 @@ __device__ __forceinline__ bf16x4 fs_act4(f32x4 acc, const float *b, bool in)
 }
 
 // Byte offset, within a ring row, of 16-byte chunk `chunk` of column x.
+// The chunk is XORed with (b1 ^ b2, b2 ^ b3) of x: the same logical chunk
+// of 16 consecutive pixels then lands on 16 distinct 16-byte bank groups of
+// a 256-byte bank line, and of any 8 consecutive pixels (x even first) on 8
+// distinct groups of a 128-byte line; it depends on x mod 16 only.
 __device__ __forceinline__ uint32_t fs_col(int x, int chunk)
 {
-    return (uint32_t)(x * 64 + ((chunk ^ ((x >> 2) & 3)) << 4));
+    const int sw = (((x >> 1) ^ (x >> 2)) & 1) | ((((x >> 2) ^ (x >> 3)) & 1) << 1);
+    return (uint32_t)(x * 64 + ((chunk ^ sw) << 4));
 }
 
 // LDS writes visible to the whole workgroup, without waiting for the
