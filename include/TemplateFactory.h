@@ -1,13 +1,22 @@
 // TemplateFactory.h -- pooled template factory.
 //
-// Same interface as okano_3_0/TemplateFactory.h:70-148.  The bilateral filter
-// needs two shapes, and those are the ones implemented:
+// Same interface as okano_3_0/TemplateFactory.h:70-148, all five shapes:
+//   TF_SHAPE_BOX      size.x by size.y box as (x, y) pairs, weights 1/(x*y),
+//                     no private parameter (okano_3_0/TemplateFactory.cu
+//                     :340-402; like the reference, only (x-1)(y-1) of the
+//                     x*y points are box points -- its loops are half-open --
+//                     and the remaining entries, unwritten there, are the
+//                     centre with weight 0 here);
+//   TF_SHAPE_CIRCLE   filled digital disc of diameter size.x (odd, >= 3,
+//                     size.y == size.x), weights 1/count (:501-666);
+//   TF_SHAPE_ARC      ring between the circles of diameters size.x < size.y
+//                     (both odd), weights 1/count (:771-936);
 //   TF_SHAPE_GAUSS    ring-ordered spatial template with Gaussian weights,
 //                     size = edge (odd, >= 3), private parameter = float sigma
 //                     (okano_3_0/TemplateFactory.cu:1057-1168);
 //   TF_SHAPE_EUCLIDE  range table exp(-d^2 / (2 sigma^2)), d < size,
 //                     private parameter = float sigma (:1293-1373).
-// BOX, CIRCLE and ARC return UNIMPLEMENT.  Templates are pooled per shape in
+// Templates are pooled per shape in
 // TF_VOL_SHAPE sets of TF_SET_SHAPE entries; two requests whose sigmas
 // differ by less than 1e-6 share one template (:1171-1188).  getTemplate and
 // putTemplate are serialised by a mutex.

@@ -253,6 +253,15 @@ int bf_synth_image(unsigned char *dst, int32_t width, int32_t height,
                    int32_t nshapes, int32_t smin, int32_t smax,
                    double noise_sigma, uint64_t seed);
 
+/* Host-side shape templates (no device work): the BOX, CIRCLE and ARC
+ * vendors of okano_3_0/TemplateFactory.cu:340-402, :501-666, :771-936 (what
+ * TemplateFactory::getTemplate pools for those shapes).  Return the point
+ * count (or a negative status); with points or weights NULL only the count
+ * is returned.  points = (x, y) pairs. */
+int bf_box_template(int32_t size_x, int32_t size_y, int32_t *points, float *weights);
+int bf_circle_template(int32_t size, int32_t *points, float *weights);
+int bf_arc_template(int32_t size_x, int32_t size_y, int32_t *points, float *weights);
+
 /* ------------------------------------------------------------------
  * The stages around the filter in okano_3_0/SimpleRegionDetect.cu:287-312
  * (detectRegion: ... -> LocalCluster -> BilateralFilter -> ConnectRegion).

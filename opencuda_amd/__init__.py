@@ -108,6 +108,9 @@ def lib() -> ctypes.CDLL:
         "bf_local_cluster": ([VP, I64, I64, VP, ctypes.POINTER(LocalClusterParams), P],
                              ctypes.c_int),
         "bf_connect_region": ([VP, VP, I32, I32, I32, I32, P], ctypes.c_int),
+        "bf_box_template": ([I32, I32, P, P], ctypes.c_int),
+        "bf_circle_template": ([I32, P, P], ctypes.c_int),
+        "bf_arc_template": ([I32, I32, P, P], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

@@ -87,6 +87,11 @@ def lib() -> ctypes.CDLL:
         L.orc_connect_region.argtypes = [P, I64, ctypes.c_int, ctypes.c_int, P, I64] + \
             [ctypes.c_int] * 4
         L.orc_connect_region.restype = ctypes.c_int
+        for name, nargs in (("orc_box_template", 2), ("orc_circle_template", 1),
+                            ("orc_arc_template", 2)):
+            f = getattr(L, name)
+            f.argtypes = [ctypes.c_int] * nargs + [P, P]
+            f.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -227,3 +232,18 @@ def connect_region(img: np.ndarray, roi=None, out_size=None, min_area: int = 100
     if rc != 0:
         raise ValueError(f"orc_connect_region: {rc}")
     return out
+
+
+# ------------------------------------------------------------ shape templates
+
+def shape_template(shape: str, *size):
+    """(points int32[n, 2] as (x, y), weights float32[n]) of the BOX, CIRCLE
+    or ARC vendor (tpl_oracle.c)."""
+    f = getattr(lib(), f"orc_{shape}_template")
+    n = f(*size, None, None)
+    if n < 0:
+        raise ValueError(f"illegal {shape} size {size}")
+    pts = np.zeros((n, 2), np.int32)
+    w = np.zeros(n, np.float32)
+    assert f(*size, _ptr(pts), _ptr(w)) == n
+    return pts, w

@@ -52,6 +52,39 @@ int main(int argc, char **argv)
     out("euclid_same", std::memcmp(ATTACHED_DATA(e), E.data(), 256 * 4) == 0);
     out("euclid_point5", e->tplData[10]);   // point 5 = (5, 0)
 
+    // --- BOX, CIRCLE, ARC: pooled, equal to the C-ABI generators
+    {
+        Template *bx = NULL, *bx2 = NULL, *ci = NULL, *ci2 = NULL, *ar = NULL;
+        out("get_box", TemplateFactory::getTemplate(&bx, TF_SHAPE_BOX, TFSize(5, 3)));
+        out("box_count", (long)bx->count);
+        std::vector<int32_t> bp(2 * 15);
+        std::vector<float> bw(15);
+        bf_box_template(5, 3, bp.data(), bw.data());
+        out("box_same", std::memcmp(bx->tplData, bp.data(), 15 * 8) == 0 &&
+                            std::memcmp(ATTACHED_DATA(bx), bw.data(), 15 * 4) == 0);
+        TemplateFactory::getTemplate(&bx2, TF_SHAPE_BOX, TFSize(5, 3));
+        out("box_pooled", bx2 == bx);
+        out("get_circle", TemplateFactory::getTemplate(&ci, TF_SHAPE_CIRCLE, 9));
+        const int nc = bf_circle_template(9, NULL, NULL);
+        out("circle_count_match", (long)ci->count == nc);
+        std::vector<int32_t> cp(2 * nc);
+        std::vector<float> cw(nc);
+        bf_circle_template(9, cp.data(), cw.data());
+        out("circle_same", std::memcmp(ci->tplData, cp.data(), nc * 8) == 0);
+        TemplateFactory::getTemplate(&ci2, TF_SHAPE_CIRCLE, 9);
+        out("circle_pooled", ci2 == ci);
+        out("get_arc", TemplateFactory::getTemplate(&ar, TF_SHAPE_ARC, TFSize(3, 5)));
+        out("arc_count", (long)ar->count);
+        Template *bad = NULL;
+        out("err_circle_even", TemplateFactory::getTemplate(&bad, TF_SHAPE_CIRCLE, 8));
+        out("err_arc_order", TemplateFactory::getTemplate(&bad, TF_SHAPE_ARC, TFSize(5, 3)));
+        TemplateFactory::putTemplate(bx);
+        TemplateFactory::putTemplate(bx2);
+        TemplateFactory::putTemplate(ci);
+        TemplateFactory::putTemplate(ci2);
+        TemplateFactory::putTemplate(ar);
+    }
+
     // --- errors
     Template *z = NULL;
     float zero = 0.0f;

@@ -4,8 +4,9 @@ driver tests/cpp/host_api_driver (no GPU needed):
 * TemplateFactory (okano_3_0/TemplateFactory.cu:1505-1675): GAUSS/EUCLIDE
   templates equal the C-ABI generators; a sigma within 1e-6 of a pooled one
   shares its template only inside the same hash set (:1079-1089, :1171-1188);
-  reference status codes for bad shapes/sizes/sigmas; other shapes are
-  UNIMPLEMENT.
+  reference status codes for bad shapes/sizes/sigmas; BOX/CIRCLE/ARC
+  templates equal the C-ABI generators and are pooled (a BOX request with a
+  private parameter is illegal, INVALID_DATA, as in the reference).
 * TPLT (okano_3_0/Template.cu:174-382) and 8-bit BMP (okano_3_0/Image.cu
   :178-432) round trips, ROI sub-image arithmetic (okano_3_0/Image.h:218-258).
 * BilateralFilter setters and doFilter's validation order
@@ -24,7 +25,10 @@ EXPECT = {
     "get_gauss_close": 0, "shared_close": 1, "get_gauss_far": 0, "shared_far": 0,
     "get_gauss_other_set": 0, "shared_other_set": 0,
     "get_euclid": 0, "euclid_count": 256, "euclid_same": 1, "euclid_point5": 5,
-    "err_shape": -1, "err_box": -998, "err_even": -1, "err_sigma0": -1, "err_null": -2,
+    "get_box": 0, "box_count": 15, "box_same": 1, "box_pooled": 1,
+    "get_circle": 0, "circle_count_match": 1, "circle_same": 1, "circle_pooled": 1,
+    "get_arc": 0, "arc_count": 8, "err_circle_even": -1, "err_arc_order": -1,
+    "err_shape": -1, "err_box": -1, "err_even": -1, "err_sigma0": -1, "err_null": -2,
     "err_put_null": -2,
     "tplt_write": 0, "tplt_read": 0, "tplt_count": 1681, "tplt_same_points": 1,
     "tplt_same_weights": 1, "tplt_missing": -4,
