@@ -624,6 +624,9 @@ constexpr int FS_IMG_ROWS = 64;           // image ring rows
 constexpr int FS_CHUNK = 16;              // image rows per prefetch
 constexpr int FS_PRE = 19;                // image rows loaded before the first step
 constexpr int FS_PF = 5;                  // prefetched bytes per thread of waves 4..7
+#ifndef FS_AHEAD
+#define FS_AHEAD 2                        // k-steps the hidden layers' A reads run ahead
+#endif
 constexpr int fs_w(int l) { return FS_W + 2 * (NLAYERS - 1 - l); }     // 64 62 60 58 56 54
 constexpr int fs_pitch(int l) { return (fs_w(l) + 15) / 16 * 16; }     // 64 (pixels)
 constexpr int fs_off(int l) { return l == 0 ? 0 : fs_off(l - 1) + FS_RING * fs_pitch(l - 1) * 64; }
@@ -889,32 +892,34 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 #pragma unroll
                     for (int nt = 0; nt < 2; nt++) acc[mm][nt] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
                 if (row_in) {
-                    // A fragments of k-step s+2 are read while the MFMAs of
-                    // k-step s run (three buffers; the scheduling barriers keep
-                    // the reads ahead of the MFMAs that do not need them).
+                    // A fragments of k-step s + FS_AHEAD are read while the
+                    // MFMAs of k-step s run (the scheduling barriers keep the
+                    // reads ahead of the MFMAs that do not need them).
                     uint32_t rbs[3];
 #pragma unroll
                     for (int d = 0; d < 3; d++)
                         rbs[d] = h_in_off + (uint32_t)(((r + d - 1) & (FS_RING - 1)) * h_in_pitch * 64);
-                    bf16x8 av[3][2];
+                    bf16x8 av[FS_AHEAD + 1][2];
 #pragma unroll
-                    for (int s = 0; s < 2; s++)
+                    for (int s = 0; s < FS_AHEAD; s++)
 #pragma unroll
                         for (int mm = 0; mm < 2; mm++)
                             av[s][mm] = fs_read_a(rbs[s / 3] + ha[s % 3] + mm * 1024);
 #pragma unroll
                     for (int s = 0; s < 9; s++) {
-                        if (s + 2 < 9) {
+                        if (s + FS_AHEAD < 9) {
 #pragma unroll
                             for (int mm = 0; mm < 2; mm++)
-                                av[(s + 2) % 3][mm] = fs_read_a(rbs[(s + 2) / 3] + ha[(s + 2) % 3] + mm * 1024);
+                                av[(s + FS_AHEAD) % (FS_AHEAD + 1)][mm] =
+                                    fs_read_a(rbs[(s + FS_AHEAD) / 3] + ha[(s + FS_AHEAD) % 3] + mm * 1024);
                         }
                         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int mm = 0; mm < 2; mm++)
 #pragma unroll
                             for (int nt = 0; nt < 2; nt++)
-                                acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[s][nt], av[s % 3][mm], acc[mm][nt], 0, 0, 0);
+                                acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    bw[s][nt], av[s % (FS_AHEAD + 1)][mm], acc[mm][nt], 0, 0, 0);
                         __builtin_amdgcn_sched_barrier(0);
                     }
                 }
