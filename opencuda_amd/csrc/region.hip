@@ -14,11 +14,12 @@
 // ConnectRegion (okano_3_0/ConnectRegion.cu:789-1043): 8-connected
 // components of pixels whose gray values lie in [minGray, maxGray], each
 // labelled by its minimum raster index.  The reference gets there with
-// shared-memory labelling per block and rounds of border merging; here one
-// lock-free union-find pass (atomicMin on roots, parents always smaller
-// than children, so every loop ends) over each pixel's four earlier
-// neighbours, then path compression -- the result does not depend on the
-// order the unions run in.  Areas, the area filter, the raster-order
+// shared-memory labelling per block and rounds of border merging; here each
+// 64 x 16 tile is labelled in LDS, then the pairs across tile borders are
+// joined by a lock-free union-find pass in global memory (atomicMin on
+// roots, parents always smaller than children, so every loop ends), then
+// path compression -- the result does not depend on the order the unions
+// run in.  Areas, the area filter, the raster-order
 // renumbering and the u8 output follow the reference, including its area
 // launch covering only rows below 256 * ceil(floor((H + 4) / 32) / 8)
 // (ConnectRegion.cu:45, :957-967) and never dropping a root whose counted
@@ -205,19 +206,103 @@ __global__ void cr_init_kernel(CrArgs a)
     }
 }
 
+// Tile pass: each 64 x 16 tile labels its own pixels in LDS with the same
+// lock-free union (LDS atomics), then writes every pixel's label as the
+// global index of its tile-local root.  Row-major order inside a tile is
+// the global order restricted to it, so a local root is the component's
+// smallest global index within the tile, never larger than its children.
+constexpr int kCrTw = 64, kCrTh = 16;
+
+__device__ __forceinline__ int lds_load(const int *L, int i)
+{
+    return __hip_atomic_load(L + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int l_find(const int *L, int x)
+{
+    int p = lds_load(L, x);
+    while (p != x) {
+        x = p;
+        p = lds_load(L, x);
+    }
+    return x;
+}
+
+__device__ void l_unite(int *L, int a, int b)
+{
+    while (true) {
+        a = l_find(L, a);
+        b = l_find(L, b);
+        if (a == b) return;
+        if (a < b) {
+            const int old = atomicMin(L + b, a);
+            if (old == b) return;
+            b = old;
+        } else {
+            const int old = atomicMin(L + a, b);
+            if (old == a) return;
+            a = old;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) cr_local_kernel(CrArgs a)
+{
+    __shared__ int loc[kCrTw * kCrTh];
+    __shared__ uint8_t fg[kCrTw * kCrTh];
+    const int x0 = blockIdx.x * kCrTw, y0 = blockIdx.y * kCrTh;
+    constexpr int kPer = kCrTw * kCrTh / 256;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        const int gx = x0 + (i & (kCrTw - 1)), gy = y0 + i / kCrTw;
+        fg[i] = gx < a.W && gy < a.H && cr_fg(a, a.in[(int64_t)gy * a.pitch + gx]);
+        loc[i] = i;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (!fg[i]) continue;
+        const int lx = i & (kCrTw - 1), ly = i / kCrTw;
+        if (lx > 0 && fg[i - 1]) l_unite(loc, i, i - 1);
+        if (ly > 0) {
+            if (lx > 0 && fg[i - kCrTw - 1]) l_unite(loc, i, i - kCrTw - 1);
+            if (fg[i - kCrTw]) l_unite(loc, i, i - kCrTw);
+            if (lx + 1 < kCrTw && fg[i - kCrTw + 1]) l_unite(loc, i, i - kCrTw + 1);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int i = (int)threadIdx.x + 256 * k;
+        const int gx = x0 + (i & (kCrTw - 1)), gy = y0 + i / kCrTw;
+        if (gx >= a.W || gy >= a.H) continue;
+        const int root = l_find(loc, i);
+        a.label[gy * a.W + gx] = (y0 + root / kCrTw) * a.W + x0 + (root & (kCrTw - 1));
+    }
+}
+
+// Border pass: the pairs of 8-neighbours in different tiles.  Every such
+// pair has one pixel whose earlier neighbour (W, NW, N or NE) lies across
+// its tile's top row, left column or right column.
 __global__ void __launch_bounds__(256) cr_union_kernel(CrArgs a)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= a.W || y >= a.H) return;
+    const bool left = (x & (kCrTw - 1)) == 0, right = (x & (kCrTw - 1)) == kCrTw - 1;
+    const bool top = (y & (kCrTh - 1)) == 0;
+    if (!(left || right || top)) return;
     if (!cr_fg(a, a.in[(int64_t)y * a.pitch + x])) return;
     const int idx = y * a.W + x;
-    if (x > 0 && cr_fg(a, a.in[(int64_t)y * a.pitch + x - 1])) cr_unite(a.label, idx, idx - 1);
+    if (left && x > 0 && cr_fg(a, a.in[(int64_t)y * a.pitch + x - 1])) cr_unite(a.label, idx, idx - 1);
     if (y > 0) {
         const uint8_t *up = a.in + (int64_t)(y - 1) * a.pitch;
-        if (x > 0 && cr_fg(a, up[x - 1])) cr_unite(a.label, idx, idx - a.W - 1);
-        if (cr_fg(a, up[x])) cr_unite(a.label, idx, idx - a.W);
-        if (x + 1 < a.W && cr_fg(a, up[x + 1])) cr_unite(a.label, idx, idx - a.W + 1);
+        if ((left || top) && x > 0 && cr_fg(a, up[x - 1])) cr_unite(a.label, idx, idx - a.W - 1);
+        if (top && cr_fg(a, up[x])) cr_unite(a.label, idx, idx - a.W);
+        if ((top || right) && x + 1 < a.W && cr_fg(a, up[x + 1]))
+            cr_unite(a.label, idx, idx - a.W + 1);
     }
 }
 
@@ -446,6 +531,8 @@ int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t
     const int flat = (int)std::min<size_t>((n + 255) / 256, 65536);
     const dim3 g2((W + 63) / 64, (H + 3) / 4);
     hipLaunchKernelGGL(cr_init_kernel, dim3(flat), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(cr_local_kernel, dim3((W + kCrTw - 1) / kCrTw, (H + kCrTh - 1) / kCrTh),
+                       dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_union_kernel, g2, dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_compress_kernel, dim3(flat), dim3(256), 0, s, a);
     if (a.covered > 0)
