@@ -116,10 +116,10 @@ def c2(steps: int = 5, mode: int = MODE_REF, tune_table: str | None = None):
     }
 
 
-def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD):
+def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD, streams: int = 4):
     import torch
     from .mixed import measure
-    out, mx = measure(count, steps, mode, log=_log)
+    out, mx = measure(count, steps, mode, log=_log, streams=streams)
     stream = torch.cuda.current_stream()
     jobs = [Job(view(mx.tin[i], it.roi), view(mx.tout[i]),
                 mx.luts[(it.sigma_space, it.sigma_range)].handle, it.radius, mode)
@@ -210,6 +210,7 @@ def _main() -> int:
     ap.add_argument("--count", type=int, default=512, help="c4: number of images")
     ap.add_argument("--mode", choices=["ref", "std"], default=None)
     ap.add_argument("--tune", default=None, help="c2: also time with this autotuner table")
+    ap.add_argument("--streams", type=int, default=4, help="c4: streams for the surrogate images")
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
@@ -222,7 +223,7 @@ def _main() -> int:
         res = c2(args.steps or 5, mode, args.tune)
     else:
         mode = MODE_REF if args.mode == "ref" else MODE_STD
-        res = c4(args.count, args.steps or 3, mode)
+        res = c4(args.count, args.steps or 3, mode, args.streams)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -4,8 +4,11 @@ with exact fallback for large radii, all in one captured graph.
 Images with r >= 9 (here r = 15, the radius the shipped network is trained
 for) run bf_sur_filter_async (surrogate + sampled exact check + exact
 fallback of failing 64x64 tiles); the others run in one persistent
-bf_batch launch.  The whole sequence is captured once into a graph and
-replayed.  Every image uses the same filter mode (STD by default: the mode
+bf_batch launch.  The batch and the surrogate images are spread over
+several streams forked from the launching stream (each stream with its own
+network instance, whose check/fallback workspace it uses), so small images
+run side by side instead of one launch after another; the whole sequence is
+captured once into a graph and replayed.  Every image uses the same filter mode (STD by default: the mode
 the network approximates), so "exact" means the same thing for both groups.
 
     python -m opencuda_amd.mixed [--count 512] [--steps 3]
@@ -26,12 +29,22 @@ from .surrogate import DEFAULT_WEIGHTS, Surrogate
 
 class Mixed:
     def __init__(self, items, net: Surrogate, mode: int = MODE_STD, sur_min_radius: int = 9,
-                 tol: int = 2, log=None):
+                 tol: int = 2, streams: int = 4, log=None):
         import torch
         self.items = items
         self.mode = mode
         self.tol = tol
         self.net = net
+        # one network instance (same weights) per stream: each owns the
+        # check/fallback workspace its calls use
+        params = net.get_params()
+        self.nets = [net]
+        for _ in range(max(1, streams) - 1):
+            n = Surrogate(seed=1)
+            n.set_params(params)
+            self.nets.append(n)
+        self.streams = [torch.cuda.Stream() for _ in range(len(self.nets))]
+        self.batch_stream = torch.cuda.Stream()
         self.luts = {}
         self.tin, self.tout = [], []
         self.sur_idx, self.ex_idx = [], []
@@ -63,15 +76,25 @@ class Mixed:
         return sorted(self.sur_idx, key=lambda i: -self.items[i].pixels)
 
     def run(self, stream=None):
-        """The sequence the graph holds: batch launch, then each surrogate image."""
+        """The sequence the graph holds: the batch launch on its own stream,
+        the surrogate images round-robin over the network streams, all
+        forked from and joined back into `stream` (default: current)."""
+        import torch
+        main = stream if stream is not None else torch.cuda.current_stream()
+        side = [self.batch_stream] + self.streams
+        for st in side:
+            st.wait_stream(main)
         if self.batch is not None:
-            self.batch.run(stream)
-        for i in self._sur_order():
+            self.batch.run(self.batch_stream)
+        for k, i in enumerate(self._sur_order()):
             it = self.items[i]
-            self.net.filter_async(view(self.tin[i], it.roi), view(self.tout[i]),
-                                  self.luts[(it.sigma_space, it.sigma_range)], it.radius,
-                                  self.mode, it.sigma_range, self.tol, self.fallback, self.tiles,
-                                  stream)
+            j = k % len(self.nets)
+            self.nets[j].filter_async(view(self.tin[i], it.roi), view(self.tout[i]),
+                                      self.luts[(it.sigma_space, it.sigma_range)], it.radius,
+                                      self.mode, it.sigma_range, self.tol, self.fallback,
+                                      self.tiles, self.streams[j])
+        for st in side:
+            main.wait_stream(st)
 
     def capture(self):
         import torch
@@ -105,11 +128,11 @@ class Mixed:
         return ref
 
 
-def measure(count=512, steps=3, mode=MODE_STD, log=None):
+def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4):
     import torch
     net = Surrogate(seed=1)
     net.load(DEFAULT_WEIGHTS)
-    mx = Mixed(wl.c4(count), net, mode=mode, log=log)
+    mx = Mixed(wl.c4(count), net, mode=mode, streams=streams, log=log)
     if log:
         log(f"{len(mx.ex_idx)} exact images, {len(mx.sur_idx)} surrogate images, "
             f"{mx.pixels / 1e6:.1f} Mpx; capturing")
@@ -137,6 +160,7 @@ def measure(count=512, steps=3, mode=MODE_STD, log=None):
         "workload": f"c4: {count} images 128..4096 per side, r in {{3,7,15}}, "
                     f"{'STD' if mode == MODE_STD else 'REF'} mode",
         "images": count, "exact_images": len(mx.ex_idx), "surrogate_images": len(mx.sur_idx),
+        "streams": len(mx.nets),
         "pixels": mx.pixels, "ms_per_graph": round(ms, 3),
         "mpix_s": round(mx.pixels / (ms / 1e3) / 1e6, 1),
         "tiles_checked": tiles // steps, "tiles_fallback": fb // steps,
@@ -153,12 +177,14 @@ def _main() -> int:
     ap.add_argument("--count", type=int, default=512)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--mode", choices=["std", "ref"], default="std")
+    ap.add_argument("--streams", type=int, default=4)
     args = ap.parse_args()
 
     def log(m):
         print(f"[mixed] {m}", file=sys.stderr, flush=True)
 
-    res, _ = measure(args.count, args.steps, MODE_STD if args.mode == "std" else MODE_REF, log=log)
+    res, _ = measure(args.count, args.steps, MODE_STD if args.mode == "std" else MODE_REF, log=log,
+                     streams=args.streams)
     print(json.dumps(res), flush=True)
     return 0
 
