@@ -185,6 +185,26 @@ static int tiles_of(const DevJob &j, int config)
     return j.tiles_x * ((j.H + 4 * tc.tgy - 1) / (4 * tc.tgy));
 }
 
+// Switch a job to the canonical per-q table of the unrolled kernels when
+// the template is the canonical ring, it holds the radius's taps and the
+// table fits next to a `config` tile.
+static bool to_ring_job(const Lut *L, int radius, int mode, int config, DevJob *job)
+{
+    const int rr = ring_rows(radius, mode);
+    if (!L->canonical || rr <= 0 || ntaps_for(radius, mode) > L->count || job->sep ||
+        lds_bytes(config, halo_class(radius), rr + 1) > max_lds_bytes())
+        return false;
+    job->rowg = L->d_rowq;
+    job->nrows = rr;
+    job->lut_rows = rr + 1;
+    return true;
+}
+
+bool ring64_job(const Lut *L, int radius, int mode, DevJob *job)
+{
+    return ring64_available(radius) && to_ring_job(L, radius, mode, 0, job);
+}
+
 static bool views_overlap(const bf_image_view *a, const bf_image_view *b)
 {
     const uint8_t *a0 = a->data, *a1 = a->data + (int64_t)(a->height - 1) * a->pitch + a->width;
@@ -427,13 +447,8 @@ int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int3
         DevJob job;
         int rc = make_job(in, out, L, radius, mode, rcfg, halo_class(radius), 0, &job, nullptr);
         if (rc) return rc;
-        const int rr = ring_rows(radius, mode);
-        if (!job.sep && lds_bytes(rcfg, halo_class(radius), rr + 1) <= max_lds_bytes()) {
-            job.rowg = L->d_rowq;
-            job.nrows = rr;
-            job.lut_rows = rr + 1;
+        if (to_ring_job(L, radius, mode, rcfg, &job))
             return launch_ring(job, radius, mode, tiles_of(job, rcfg), (hipStream_t)stream);
-        }
     }
     const int rows = fused_rows(lut_from_handle(lut), radius, mode);
     const int tuned = tuned_config(radius, mode);

@@ -127,48 +127,68 @@ struct Geo {
     static constexpr int NT = TW * TGY;
 };
 
-// Stage the clamped input tile (rows Y0-HALO .., cols X0-HALO ..) into LDS
-// as byte-replicated u32.
-template <int TW, int TGY, int HALO>
-__device__ __forceinline__ void load_tile(const DevJob &job, int X0, int Y0, int tid)
+typedef __attribute__((address_space(3))) uint8_t lds_byte;
+
+__device__ __forceinline__ uint32_t lds_u8(uint32_t byte_addr)
 {
-    using G = Geo<TW, TGY, HALO>;
-    const int xs = X0 - HALO;
-    const int ys = Y0 - HALO;
+    return *(const lds_byte *)(size_t)byte_addr;
+}
+
+// Stage the clamped input rectangle (rows ys .. ys+ROWS-1, columns xs ..
+// xs+COLS-1, clamped to the input view) into LDS at `base`, row-major with
+// COLS elements per row: byte-replicated u32 elements (the exact kernels'
+// v_sad_u8 form) or, with BYTES, plain bytes.  NT threads (ids 0..NT-1)
+// share the work.
+template <int COLS, int ROWS, int NT, bool BYTES>
+__device__ __forceinline__ void stage_rect(const DevJob &job, int xs, int ys, uint32_t base,
+                                           int tid)
+{
+    constexpr uint32_t ESZ = BYTES ? 1u : 4u;
     const int wmax = job.in_w - 1, hmax = job.in_h - 1;
-    if (xs >= 0 && xs + G::TCOLS <= job.in_w) {
-        // Interior columns: aligned 16-byte loads covering [xs, xs+TCOLS).
+    auto put = [&](int e, uint32_t byte) {
+        if (BYTES)
+            *(lds_byte *)(size_t)(base + (uint32_t)e) = (uint8_t)byte;
+        else
+            lds_store_u32(base + (uint32_t)e * ESZ, byte * 0x01010101u);
+    };
+    if (xs >= 0 && xs + COLS <= job.in_w) {
+        // Interior columns: aligned 16-byte loads covering [xs, xs+COLS).
         // Bytes outside that span are dropped; they lie in the same aligned
         // 16-byte block as a byte of the row, hence in mapped memory.
-        constexpr int NCH = (G::TCOLS + 15) / 16 + 1;  // chunks per row
-        for (int e = tid; e < G::TROWS * NCH; e += G::NT) {
+        constexpr int NCH = (COLS + 15) / 16 + 1;  // chunks per row
+        for (int e = tid; e < ROWS * NCH; e += NT) {
             const int r = e / NCH, ch = e - r * NCH;
             const int gy = clampi(ys + r, 0, hmax);
             const uint8_t *row = job.in + (int64_t)gy * job.in_pitch;
             const uintptr_t a0 = reinterpret_cast<uintptr_t>(row + xs) & ~(uintptr_t)15;
             const uintptr_t ca = a0 + 16 * (uintptr_t)ch;
             const intptr_t first = (intptr_t)ca - (intptr_t)reinterpret_cast<uintptr_t>(row + xs);
-            if (first >= G::TCOLS) continue;
+            if (first >= COLS) continue;
             const uint4 v = *reinterpret_cast<const uint4 *>(ca);
             const uint32_t w[4] = { v.x, v.y, v.z, v.w };
 #pragma unroll
             for (int b = 0; b < 16; b++) {
                 const int c = (int)first + b;
-                if (c >= 0 && c < G::TCOLS) {
-                    const uint32_t byte = (w[b >> 2] >> (8 * (b & 3))) & 0xFFu;
-                    lds_store_u32(kTileOff + (uint32_t)((r * G::TCOLS + c) * 4), byte * 0x01010101u);
-                }
+                if (c >= 0 && c < COLS) put(r * COLS + c, (w[b >> 2] >> (8 * (b & 3))) & 0xFFu);
             }
         }
     } else {
-        for (int e = tid; e < G::TROWS * G::TCOLS; e += G::NT) {
-            const int r = e / G::TCOLS, c = e - r * G::TCOLS;
+        for (int e = tid; e < ROWS * COLS; e += NT) {
+            const int r = e / COLS, c = e - r * COLS;
             const int gy = clampi(ys + r, 0, hmax);
             const int gx = clampi(xs + c, 0, wmax);
-            const uint32_t byte = job.in[(int64_t)gy * job.in_pitch + gx];
-            lds_store_u32(kTileOff + (uint32_t)(e * 4), byte * 0x01010101u);
+            put(e, job.in[(int64_t)gy * job.in_pitch + gx]);
         }
     }
+}
+
+// Stage the clamped input tile (rows Y0-HALO .., cols X0-HALO ..) into LDS
+// as byte-replicated u32.
+template <int TW, int TGY, int HALO>
+__device__ __forceinline__ void load_tile(const DevJob &job, int X0, int Y0, int tid)
+{
+    using G = Geo<TW, TGY, HALO>;
+    stage_rect<G::TCOLS, G::TROWS, G::NT, false>(job, X0 - HALO, Y0 - HALO, kTileOff, tid);
 }
 
 // Fused weight table: W[row][d] = rowg[row] * range[d]; row `nrows` = 0.
@@ -483,16 +503,20 @@ __device__ __forceinline__ void tile_ring(const DevJob &job, int X0, int Y0, int
     }
 }
 
-// One image, static grid-stride over its tiles; job.rowg holds the
+// One image, static grid-stride over its tiles (or over the tiles of a
+// device list, whose length is read on the device); job.rowg holds the
 // per-q weights of the canonical template and job.nrows their count.
 template <int R, int MODE, int TW, int TGY, int HALO>
-__global__ void __launch_bounds__(TW * TGY) bf_ring_kernel(DevJob job, int ntiles)
+__global__ void __launch_bounds__(TW * TGY)
+bf_ring_kernel(DevJob job, int ntiles, const int32_t *list, const int32_t *list_count)
 {
     using G = Geo<TW, TGY, HALO>;
     const int tid = threadIdx.x;
     const int lx = tid % TW, gy = tid / TW;
     build_lut<G::NT>(job, kTileOff + (uint32_t)G::TILE_BYTES, tid);
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int n = list ? *as_const(list_count) : ntiles;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int tile = list ? as_const(list)[i] : i;
         const int ty = tile / job.tiles_x;
         const int X0 = (tile - ty * job.tiles_x) * TW;
         const int Y0 = ty * G::TH;
@@ -500,6 +524,157 @@ __global__ void __launch_bounds__(TW * TGY) bf_ring_kernel(DevJob job, int ntile
         load_tile<TW, TGY, HALO>(job, X0, Y0, tid);
         __syncthreads();
         tile_ring<R, MODE, TW, TGY, HALO>(job, X0, Y0, lx, gy);
+    }
+}
+
+// ------------------------------------------------------ exact sample check
+//
+// For each 64x64 tile of the output view, 64 samples (one per 8x8 cell at a
+// hashed position) are filtered exactly -- same taps, order and rounding as
+// the exact kernels -- and compared with the u8 already in the output (the
+// surrogate's); a tile with any sampled |difference| > tol is appended to
+// the fallback list.
+
+__device__ __forceinline__ uint32_t hash32(uint32_t x)
+{
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+
+// Sample of cell `cell` (0..63) of tile `tile`: offsets inside the tile.
+__device__ __forceinline__ void check_sample(int tile, int cell, uint32_t salt, int *sx, int *sy)
+{
+    const uint32_t h = hash32((uint32_t)tile * 64u + (uint32_t)cell + salt);
+    *sx = (cell & 7) * 8 + (int)(h & 7);
+    *sy = (cell >> 3) * 8 + (int)((h >> 3) & 7);
+}
+
+__device__ __forceinline__ void check_verdict(const CheckArgs &a, int tile, bool bad)
+{
+    if (__any(bad) && (threadIdx.x & 63) == 0) {
+        a.flags[tile] = 1;
+        const int slot = atomicAdd(a.count, 1);
+        a.list[slot] = tile;
+        if (a.accum) atomicAdd(a.accum, 1);
+    }
+    if ((threadIdx.x & 63) == 0 && a.tiles_accum) atomicAdd(a.tiles_accum, 1);
+}
+
+// Any template: one wave per tile, one sample per lane.  The tile, its
+// context rows and columns (clamped to the input view, like the exact
+// kernels) and the range table are staged in LDS; each lane then runs the
+// exact tap loop of the run-time tap list.
+constexpr int kCheckHaloMax = 20;     // largest halo class
+constexpr int kCheckSpanMax = 64 + 2 * kCheckHaloMax;
+
+__global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
+{
+    __shared__ uint8_t win[kCheckSpanMax * kCheckSpanMax];
+    __shared__ float rng[256];
+    const int tile = blockIdx.x;
+    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+    const DevJob &j = a.job;
+    const int halo = a.halo, span = 64 + 2 * halo;
+    const int gx0 = tx * 64 - halo, gy0 = ty * 64 - halo;
+    for (int i = threadIdx.x; i < 256; i += 64) rng[i] = j.range[i];
+    for (int yy = 0; yy < span; yy++) {
+        const int gy = clampi(gy0 + yy, 0, j.in_h - 1);
+        const unsigned char *src = j.in + (int64_t)gy * j.in_pitch;
+        for (int xx = threadIdx.x; xx < span; xx += 64)
+            win[yy * span + xx] = src[clampi(gx0 + xx, 0, j.in_w - 1)];
+    }
+    __syncthreads();
+    int sx, sy;
+    check_sample(tile, threadIdx.x, a.salt, &sx, &sy);
+    const int x = tx * 64 + sx, y = ty * 64 + sy;
+    bool bad = false;
+    if (x < j.W && y < j.H) {
+        const int y0 = j.mode == kModeRef ? y - (y & 3) : y;
+        const int base = (y0 - gy0) * span + (x - gx0);
+        const uint32_t c = win[(y - gy0) * span + (x - gx0)];
+        float t = 0.0f, s = 0.0f;
+        for (int k = 0; k < j.ntaps_padded; k++) {
+            const uint32_t info = j.taps[k];
+            const int dy = (int)(int8_t)(info & 0xFFu), dx = (int)(int8_t)((info >> 8) & 0xFFu);
+            const int row = (int)(info >> 16);
+            const uint32_t p = win[base + dy * span + dx];
+            const uint32_t d = p > c ? p - c : c - p;
+            const float g = row < j.nrows ? a.rowg[row] : 0.0f;
+            const float f = g * rng[d];
+            t = t + f * (float)p;
+            s = s + f;
+        }
+        const int ex = to_u8(t / s);
+        const int su = j.out[(int64_t)y * j.out_pitch + x];
+        bad = (ex - su > a.tol) || (su - ex > a.tol);
+    }
+    check_verdict(a, tile, bad);
+}
+
+// The canonical ring template, unrolled: a persistent workgroup of four
+// waves stages the range table E[d] and the per-q weights g once, then each
+// wave takes one tile at a time, stages its context as bytes in its own LDS
+// window and runs the compiled tap list for its 64 samples (tap offsets and
+// weight rows are immediates, as in bf_ring_kernel).  The factor is formed
+// per tap as g * E[d] -- the product the fused tables hold, with the same
+// single rounding -- so LDS holds 4 windows + 1 KB + the weights, and
+// several workgroups share a CU (the taps' LDS latency needs the waves).
+#ifndef BF_CHECK_GROUP
+#define BF_CHECK_GROUP 8                  // taps per scheduling window of ring_check_kernel
+#endif
+
+template <int R, int MODE, int HALO>
+__global__ void __launch_bounds__(256) ring_check_kernel(CheckArgs a, int ntiles)
+{
+    constexpr int SPAN = 64 + 2 * HALO;
+    constexpr uint32_t WIN = SPAN * SPAN;
+    constexpr uint32_t E_BASE = kTileOff + 4 * WIN;
+    constexpr uint32_t G_BASE = E_BASE + 1024;
+    constexpr int T = RingTaps<R, MODE>::T;
+    constexpr int MINB = ring_min_offset<R, MODE, SPAN>();
+    const DevJob &j = a.job;
+    const int tid = threadIdx.x, wave = tid >> 6, cell = tid & 63;
+    const uint32_t win = kTileOff + (uint32_t)wave * WIN;
+    for (int d = tid; d < 256; d += 256) lds_store_u32(E_BASE + 4 * d, __float_as_uint(j.range[d]));
+    for (int r = tid; r < j.nrows; r += 256) lds_store_u32(G_BASE + 4 * r, __float_as_uint(j.rowg[r]));
+    const int groups = (ntiles + 3) / 4;
+    for (int g = blockIdx.x; g < groups; g += gridDim.x) {
+        const int tile = 4 * g + wave;
+        const bool live = tile < ntiles;
+        const int X0 = live ? tile % a.tiles_x * 64 : 0, Y0 = live ? tile / a.tiles_x * 64 : 0;
+        __syncthreads();                      // tables staged / previous windows consumed
+        if (live) stage_rect<SPAN, SPAN, 64, true>(j, X0 - HALO, Y0 - HALO, win, cell);
+        __syncthreads();
+        if (!live) continue;
+        int sx, sy;
+        check_sample(tile, cell, a.salt, &sx, &sy);
+        const int x = X0 + sx, y = Y0 + sy;
+        bool bad = false;
+        if (x < j.W && y < j.H) {
+            // REF: the taps are centred on the first row of the sample's
+            // 4-row group, the range weights on the sample's own pixel
+            const int wy = MODE == kModeRef ? sy - (sy & 3) : sy;
+            const uint32_t c4 = lds_u8(win + (uint32_t)((sy + HALO) * SPAN + sx + HALO)) * 0x01010101u;
+            const uint32_t tbase = win + (uint32_t)((wy + HALO) * SPAN + sx + HALO + MINB);
+            float t = 0.0f, s = 0.0f;
+            static_for<T>([&](auto ci) {
+                constexpr RingTap tp = kRing<R, MODE>.t[decltype(ci)::value];
+                constexpr int TAPB = tp.dy * SPAN + tp.dx - MINB;   // >= 0
+                const uint32_t p = lds_u8(tbase + TAPB);
+                // v_perm_b32 with selector 0 replicates byte 0 (one full-rate op)
+                const uint32_t p4 = __builtin_amdgcn_perm(p, p, 0u);
+                const float e = lds_f32(__builtin_amdgcn_sad_u8(p4, c4, E_BASE));
+                const float f = lds_f32(G_BASE + 4u * (uint32_t)tp.row) * e;
+                t = t + f * (float)(p4 & 0xFFu);
+                s = s + f;
+                if constexpr (decltype(ci)::value % BF_CHECK_GROUP == BF_CHECK_GROUP - 1)
+                    asm volatile("" : "+v"(t), "+v"(s)::"memory");
+            });
+            const int ex = to_u8(t / s);
+            const int su = j.out[(int64_t)y * j.out_pitch + x];
+            bad = (ex - su > a.tol) || (su - ex > a.tol);
+        }
+        check_verdict(a, tile, bad);
     }
 }
 
@@ -518,27 +693,43 @@ constexpr TileConfig kConfigs[] = {
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kHalos[] = { 4, 8, 12, 16, 20 };
 
+// hipFuncSetAttribute once per kernel (thread-safe static initialisation):
+// allow the dynamic LDS of a kernel to reach the CU's 160 KB.
+template <auto K>
+hipError_t allow_full_lds()
+{
+    static const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(K),
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    160 * 1024);
+    return e;
+}
+
+// Workgroups of `kern` (NT threads, lds bytes) that run at once on the
+// device; 0 when one does not fit.
+int resident_groups(const void *kern, int nt, size_t lds)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nt, lds) != hipSuccess)
+        return -1;
+    return cus * per_cu;
+}
+
 template <int TW, int TGY, int HALO>
 int launch_one(const KArgs &a, int max_lut_rows, hipStream_t s)
 {
     using G = Geo<TW, TGY, HALO>;
     const size_t lds = kTileOff + (size_t)G::TILE_BYTES + (size_t)max_lut_rows * 1024;
-    auto kern = bf_exact_kernel<TW, TGY, HALO>;
-    static int configured = 0;   // per instantiation
-    if (!configured) {
-        BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        configured = 1;
-    }
-    int dev = 0, cus = 0, per_cu = 0;
-    BF_HIP(hipGetDevice(&dev));
-    BF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G::NT, lds));
-    if (per_cu < 1) {
+    constexpr auto kern = bf_exact_kernel<TW, TGY, HALO>;
+    BF_HIP(allow_full_lds<kern>());
+    const int res = resident_groups(reinterpret_cast<const void *>(kern), G::NT, lds);
+    if (res < 0) return hip_fail(hipGetLastError(), "occupancy query");
+    if (res < 1) {
         set_error("exact kernel does not fit on a CU (LDS " + std::to_string(lds) + " bytes)");
         return kInvalidData;
     }
-    int grid = cus * per_cu;
+    int grid = res;
     if (!a.tile_list && !a.counter && a.ntiles < grid) grid = a.ntiles;
     if (a.tile_list && a.ntiles < grid) grid = a.ntiles;   // ntiles = max possible
     if (grid < 1) return kOk;
@@ -633,28 +824,42 @@ int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_max,
 namespace {
 
 template <int R, int MODE, int TW, int TGY, int HALO>
-int launch_ring_one(const DevJob &job, int ntiles, hipStream_t s)
+int launch_ring_one(const DevJob &job, int ntiles, const int32_t *list, const int32_t *count,
+                    hipStream_t s)
 {
     using G = Geo<TW, TGY, HALO>;
+    constexpr auto kern = bf_ring_kernel<R, MODE, TW, TGY, HALO>;
     const size_t lds = kTileOff + (size_t)G::TILE_BYTES + (size_t)(job.nrows + 1) * 1024;
-    auto kern = bf_ring_kernel<R, MODE, TW, TGY, HALO>;
-    static int configured = 0;
-    if (!configured) {
-        BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        configured = 1;
-    }
-    int dev = 0, cus = 0, per_cu = 0;
-    BF_HIP(hipGetDevice(&dev));
-    BF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    BF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, G::NT, lds));
-    if (per_cu < 1) {
+    BF_HIP(allow_full_lds<kern>());
+    const int res = resident_groups(reinterpret_cast<const void *>(kern), G::NT, lds);
+    if (res < 0) return hip_fail(hipGetLastError(), "occupancy query");
+    if (res < 1) {
         set_error("ring kernel does not fit on a CU");
         return kInvalidData;
     }
-    const int grid = std::min(ntiles, cus * per_cu);
+    const int grid = std::min(ntiles, res);   // with a list, ntiles = its maximum length
     if (grid < 1) return kOk;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(G::NT), lds, s, job, ntiles);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(G::NT), lds, s, job, ntiles, list, count);
+    BF_HIP(hipGetLastError());
+    return kOk;
+}
+
+template <int R, int MODE, int HALO>
+int launch_ring_check_one(const CheckArgs &a, int ntiles, hipStream_t s)
+{
+    constexpr auto kern = ring_check_kernel<R, MODE, HALO>;
+    const size_t lds = kTileOff + 4 * (size_t)(64 + 2 * HALO) * (64 + 2 * HALO) + 1024 +
+                       4 * (size_t)a.job.nrows;
+    BF_HIP(allow_full_lds<kern>());
+    const int res = resident_groups(reinterpret_cast<const void *>(kern), 256, lds);
+    if (res < 0) return hip_fail(hipGetLastError(), "occupancy query");
+    if (res < 1) {
+        set_error("check kernel does not fit on a CU");
+        return kInvalidData;
+    }
+    const int grid = std::min((ntiles + 3) / 4, res);
+    if (grid < 1) return kOk;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(256), lds, s, a, ntiles);
     BF_HIP(hipGetLastError());
     return kOk;
 }
@@ -691,17 +896,53 @@ int launch_ring(const DevJob &job, int radius, int mode, int ntiles, hipStream_t
 {
     const bool ref = mode == kModeRef;
     switch (radius) {
-    case 15: return ref ? launch_ring_one<15, 0, 64, 16, 16>(job, ntiles, s)
-                        : launch_ring_one<15, 1, 64, 16, 16>(job, ntiles, s);
-    case 9: return ref ? launch_ring_one<9, 0, 64, 8, 12>(job, ntiles, s)
-                       : launch_ring_one<9, 1, 64, 8, 12>(job, ntiles, s);
-    case 7: return ref ? launch_ring_one<7, 0, 64, 8, 8>(job, ntiles, s)
-                       : launch_ring_one<7, 1, 64, 8, 8>(job, ntiles, s);
-    case 5: return ref ? launch_ring_one<5, 0, 64, 8, 8>(job, ntiles, s)
-                       : launch_ring_one<5, 1, 64, 8, 8>(job, ntiles, s);
-    case 3: return ref ? launch_ring_one<3, 0, 64, 8, 4>(job, ntiles, s)
-                       : launch_ring_one<3, 1, 64, 8, 4>(job, ntiles, s);
+    case 15: return ref ? launch_ring_one<15, 0, 64, 16, 16>(job, ntiles, nullptr, nullptr, s)
+                        : launch_ring_one<15, 1, 64, 16, 16>(job, ntiles, nullptr, nullptr, s);
+    case 9: return ref ? launch_ring_one<9, 0, 64, 8, 12>(job, ntiles, nullptr, nullptr, s)
+                       : launch_ring_one<9, 1, 64, 8, 12>(job, ntiles, nullptr, nullptr, s);
+    case 7: return ref ? launch_ring_one<7, 0, 64, 8, 8>(job, ntiles, nullptr, nullptr, s)
+                       : launch_ring_one<7, 1, 64, 8, 8>(job, ntiles, nullptr, nullptr, s);
+    case 5: return ref ? launch_ring_one<5, 0, 64, 8, 8>(job, ntiles, nullptr, nullptr, s)
+                       : launch_ring_one<5, 1, 64, 8, 8>(job, ntiles, nullptr, nullptr, s);
+    case 3: return ref ? launch_ring_one<3, 0, 64, 8, 4>(job, ntiles, nullptr, nullptr, s)
+                       : launch_ring_one<3, 1, 64, 8, 4>(job, ntiles, nullptr, nullptr, s);
     default: set_error("no unrolled kernel for this radius"); return kInvalidData;
+    }
+}
+
+bool ring64_available(int radius) { return radius == 15 || radius == 9; }
+
+int ring64_tiles(const DevJob &job, int radius, int mode, const int32_t *list,
+                 const int32_t *count, int ntiles_max, hipStream_t s)
+{
+    const bool ref = mode == kModeRef;
+    switch (radius) {
+    case 15: return ref ? launch_ring_one<15, 0, 64, 16, 16>(job, ntiles_max, list, count, s)
+                        : launch_ring_one<15, 1, 64, 16, 16>(job, ntiles_max, list, count, s);
+    case 9: return ref ? launch_ring_one<9, 0, 64, 16, 12>(job, ntiles_max, list, count, s)
+                       : launch_ring_one<9, 1, 64, 16, 12>(job, ntiles_max, list, count, s);
+    default: set_error("no unrolled 64x64 kernel for this radius"); return kInvalidData;
+    }
+}
+
+int launch_check(const CheckArgs &a, int ring_radius, int ntiles, hipStream_t s)
+{
+    const bool ref = a.job.mode == kModeRef;
+    switch (ring_radius) {
+    case 15: return ref ? launch_ring_check_one<15, 0, 16>(a, ntiles, s)
+                        : launch_ring_check_one<15, 1, 16>(a, ntiles, s);
+    case 9: return ref ? launch_ring_check_one<9, 0, 12>(a, ntiles, s)
+                       : launch_ring_check_one<9, 1, 12>(a, ntiles, s);
+    case 0:
+        if (a.halo < 0 || a.halo > kCheckHaloMax) {
+            set_error("radius outside the check kernel's halo");
+            return kInvalidData;
+        }
+        if (ntiles < 1) return kOk;
+        hipLaunchKernelGGL(check_kernel, dim3(ntiles), dim3(64), 0, s, a);
+        BF_HIP(hipGetLastError());
+        return kOk;
+    default: set_error("no unrolled check kernel for this radius"); return kInvalidData;
     }
 }
 

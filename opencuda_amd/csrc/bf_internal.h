@@ -130,4 +130,30 @@ int current_device();
 int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_dev_max,
                 const int32_t *count_dev, int config, int halo, hipStream_t s);
 
+// Unrolled kernels on 64x64 tiles (the surrogate's check and fallback
+// tiles) for the canonical template at these radii.  ring64_job switches a
+// job made for config 0 to the canonical per-q table when they apply.
+bool ring64_available(int radius);
+bool ring64_job(const Lut *lut, int radius, int mode, DevJob *job);
+int ring64_tiles(const DevJob &job, int radius, int mode, const int32_t *list,
+                 const int32_t *count, int ntiles_max, hipStream_t s);
+
+// Exact sample check of 64x64 tiles (one hashed sample per 8x8 cell).
+struct CheckArgs {
+    DevJob job;                  // exact job (input view, output view, tables)
+    const float *rowg;           // weights of the tap rows (run-time tap list)
+    int tiles_x, tiles_y;
+    int tol;
+    int32_t *flags;              // tiles_x * tiles_y
+    int32_t *list;               // fallback tile ids
+    int32_t *count;
+    int32_t *accum;              // optional running total of fallback tiles
+    int32_t *tiles_accum;        // optional running total of tiles
+    uint32_t salt;
+    int halo;                    // context rows/columns staged around a tile
+};
+// ring_radius 0: the run-time tap list; 9 or 15: the unrolled canonical
+// template (job from ring64_job).
+int launch_check(const CheckArgs &a, int ring_radius, int ntiles, hipStream_t s);
+
 }  // namespace bfimpl

@@ -484,101 +484,6 @@ __global__ void pack_kernel(const float *params, PackTables pk)
     }
 }
 
-// ------------------------------------------------------ exact sample check
-
-// For each 64x64 tile of the output view, 64 samples (one per 8x8 cell at a
-// hashed position) are filtered exactly -- same taps, order and rounding as
-// bf_exact -- and compared with the surrogate's u8; a tile with any sampled
-// |difference| > tol is appended to the fallback list.
-struct CheckArgs {
-    DevJob job;                  // exact job (input view, output view, tables)
-    const float *rowg;
-    int tiles_x, tiles_y;
-    int tol;
-    int32_t *flags;              // tiles_x * tiles_y
-    int32_t *list;               // fallback tile ids
-    int32_t *count;
-    int32_t *accum;              // optional running total of fallback tiles
-    int32_t *tiles_accum;        // optional running total of tiles
-    uint32_t salt;
-    int halo;                    // context rows/columns staged around a tile
-};
-
-__device__ __forceinline__ uint32_t hash32(uint32_t x)
-{
-    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
-    return x;
-}
-
-__device__ __forceinline__ uint8_t to_u8_exact(float q)
-{
-    if (!(q > 0.0f)) return 0;
-    if (q >= 255.0f) return 255;
-    return (uint8_t)(int)q;
-}
-
-// One wave per 64 x 64 tile, one sample per lane.  The tile, its context
-// rows and columns (clamped to the input view, like the exact kernels) and
-// the range table are staged in LDS; each lane then runs the exact tap loop
-// in the exact kernels' order, so its value is the exact filter's.
-constexpr int kCheckHaloMax = 20;     // largest halo class
-constexpr int kCheckSpanMax = 64 + 2 * kCheckHaloMax;
-
-__global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
-{
-    __shared__ uint8_t win[kCheckSpanMax * kCheckSpanMax];
-    __shared__ float rng[256];
-    const int tile = blockIdx.x;
-    const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-    const DevJob &j = a.job;
-    const int halo = a.halo, span = 64 + 2 * halo;
-    const int gx0 = tx * 64 - halo, gy0 = ty * 64 - halo;
-    for (int i = threadIdx.x; i < 256; i += 64) rng[i] = j.range[i];
-    for (int yy = 0; yy < span; yy++) {
-        int gy = gy0 + yy;
-        gy = gy < 0 ? 0 : (gy >= j.in_h ? j.in_h - 1 : gy);
-        const unsigned char *src = j.in + (int64_t)gy * j.in_pitch;
-        for (int xx = threadIdx.x; xx < span; xx += 64) {
-            int gx = gx0 + xx;
-            gx = gx < 0 ? 0 : (gx >= j.in_w ? j.in_w - 1 : gx);
-            win[yy * span + xx] = src[gx];
-        }
-    }
-    __syncthreads();
-    const int cell = threadIdx.x;            // 8 x 8 cells of 8 x 8 pixels
-    const uint32_t h = hash32((uint32_t)tile * 64u + (uint32_t)cell + a.salt);
-    const int x = tx * 64 + (cell & 7) * 8 + (int)(h & 7);
-    const int y = ty * 64 + (cell >> 3) * 8 + (int)((h >> 3) & 7);
-    bool bad = false;
-    if (x < j.W && y < j.H) {
-        const int y0 = j.mode == BF_MODE_REF ? y - (y & 3) : y;
-        const int base = (y0 - gy0) * span + (x - gx0);
-        const uint32_t c = win[(y - gy0) * span + (x - gx0)];
-        float t = 0.0f, s = 0.0f;
-        for (int k = 0; k < j.ntaps_padded; k++) {
-            const uint32_t info = j.taps[k];
-            const int dy = (int)(int8_t)(info & 0xFFu), dx = (int)(int8_t)((info >> 8) & 0xFFu);
-            const int row = (int)(info >> 16);
-            const uint32_t p = win[base + dy * span + dx];
-            const uint32_t d = p > c ? p - c : c - p;
-            const float g = row < j.nrows ? a.rowg[row] : 0.0f;
-            const float f = g * rng[d];
-            t = t + f * (float)p;
-            s = s + f;
-        }
-        const int ex = to_u8_exact(t / s);
-        const int su = j.out[(int64_t)y * j.out_pitch + x];
-        bad = (ex - su > a.tol) || (su - ex > a.tol);
-    }
-    if (__any(bad) && threadIdx.x == 0) {
-        a.flags[tile] = 1;
-        const int slot = atomicAdd(a.count, 1);
-        a.list[slot] = tile;
-        if (a.accum) atomicAdd(a.accum, 1);
-    }
-    if (threadIdx.x == 0 && a.tiles_accum) atomicAdd(a.tiles_accum, 1);
-}
-
 // ------------------------------------------------------- fused inference
 //
 // All six layers of the forward pass in one kernel, for inference: no
@@ -1420,6 +1325,10 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     hipLaunchKernelGGL(zero_ints, dim3((nt + 255) / 256), dim3(256), 0, s, net->flags, nt);
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, net->count, 1);
     BF_HIP(hipGetLastError());
+    // The canonical template at r = 9 or 15 runs the unrolled 64x64-tile
+    // kernels for both the check and the fallback (same taps, order and
+    // rounding as the run-time tap list, so the same verdicts and pixels).
+    const bool ring = ring64_job(lut_from_handle(lut), radius, mode, &job);
     CheckArgs ca{};
     ca.job = job;
     ca.rowg = lut_from_handle(lut)->d_rowg;
@@ -1435,13 +1344,9 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     // make_job has checked every tap against this halo (REF rows may reach
     // two further down, which the 4-row grouping of the sample row covers).
     ca.halo = halo_class(radius);
-    if (ca.halo < 0 || ca.halo > kCheckHaloMax) {
-        set_error("radius outside the check kernel's halo");
-        return kInvalidData;
-    }
-    hipLaunchKernelGGL(check_kernel, dim3(nt), dim3(64), 0, s, ca);
-    BF_HIP(hipGetLastError());
+    if ((rc = launch_check(ca, ring ? radius : 0, nt, s))) return rc;
     // exact fallback over the listed tiles
+    if (ring) return ring64_tiles(job, radius, mode, net->list, net->count, nt, s);
     return exact_tiles(job, net->list, nt, net->count, 0, halo_class(radius), s);
 }
 

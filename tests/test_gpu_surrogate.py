@@ -8,6 +8,9 @@
   per-layer kernels, bit for bit, on ragged, multi-band and pitched images;
 * the check-and-fallback path: tolerance -1 makes every tile exact
   (bit-exact to bf_exact), tolerance 255 leaves the pure surrogate output;
+  at tolerance 2 the fallback set is exactly the tiles whose 64 hashed
+  samples (restated here) hold a difference > 2, for the unrolled check of
+  the canonical template (r = 9, 15) and the run-time tap list;
 * the trained network against the method without it (identity / untrained
   network): what fraction of pixels it gets within 2 LSB of the exact
   filter, and how many 64x64 tiles survive the sampled check.
@@ -18,6 +21,7 @@ import numpy as np
 import pytest
 
 import opencuda_amd as oc
+import oracle
 from opencuda_amd import surrogate as sg
 
 pytestmark = pytest.mark.gpu
@@ -160,6 +164,73 @@ def test_no_fallback_is_pure_surrogate(device):
     net.infer(oc.view(tin), oc.view(b), 30.0)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+def _hash32(x):
+    m = 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7feb352d) & m
+    x ^= x >> 15
+    x = (x * 0x846ca68b) & m
+    x ^= x >> 16
+    return x
+
+
+def _expected_fallback(sur, ex, tol, salt=0x5bd1e995):
+    """Tiles whose sampled |surrogate - exact| exceeds tol: one sample per
+    8x8 cell of each 64x64 tile at a hashed position."""
+    h, w = ex.shape
+    tx, ty = (w + 63) // 64, (h + 63) // 64
+    bad = np.zeros(tx * ty, bool)
+    for tile in range(tx * ty):
+        X0, Y0 = tile % tx * 64, tile // tx * 64
+        for cell in range(64):
+            hv = _hash32((tile * 64 + cell + salt) & 0xFFFFFFFF)
+            x = X0 + (cell & 7) * 8 + (hv & 7)
+            y = Y0 + (cell >> 3) * 8 + ((hv >> 3) & 7)
+            if x < w and y < h and abs(int(sur[y, x]) - int(ex[y, x])) > tol:
+                bad[tile] = True
+    return bad, tx
+
+
+@pytest.mark.parametrize("radius,mode,table", [
+    (15, oc.MODE_STD, "gauss"),      # unrolled check + unrolled fallback
+    (15, oc.MODE_REF, "gauss"),
+    (9, oc.MODE_STD, "gauss"),
+    (11, oc.MODE_STD, "gauss"),      # run-time tap list
+    (15, oc.MODE_STD, "custom"),     # not the canonical template: run-time tap list
+])
+def test_check_flags_exactly_the_sampled_tiles(device, radius, mode, table):
+    import torch
+    net = sg.Surrogate(seed=1)
+    if os.path.exists(sg.DEFAULT_WEIGHTS):
+        net.load(sg.DEFAULT_WEIGHTS)
+    H, W = 300, 457
+    img = oc.synth_image(W, H, 30, 8, 120, 12.0, 40 + radius)
+    if table == "gauss":
+        lut = oc.Lut.from_sigmas(10.0, 30.0)
+    else:
+        off, wt = oracle.gauss_template(10.0)
+        wt = wt.copy()
+        wt[5] *= np.float32(0.5)          # one ring point weighted apart from its q
+        lut = oc.Lut(off, wt, oracle.euclid_template(30.0))
+    tin = torch.from_numpy(img).cuda()
+    sur, ex, out = (torch.zeros_like(tin) for _ in range(3))
+    net.infer(oc.view(tin), oc.view(sur), 30.0)
+    oc.exact(oc.view(tin), oc.view(ex), lut, radius, mode)
+    total, fb = net.filter(oc.view(tin), oc.view(out), lut, radius, mode, 30.0, tol_lsb=2)
+    torch.cuda.synchronize()
+    s_np, e_np, o_np = sur.cpu().numpy(), ex.cpu().numpy(), out.cpu().numpy()
+    bad, tx = _expected_fallback(s_np, e_np, 2)
+    assert total == bad.size and fb == int(bad.sum())
+    # the network approximates STD: in REF mode (4-row windows) every tile
+    # may fail, in STD some must pass
+    assert fb > 0 and (mode == oc.MODE_REF or fb < total)
+    exp = s_np.copy()
+    for tile in np.flatnonzero(bad):
+        X0, Y0 = tile % tx * 64, tile // tx * 64
+        exp[Y0:Y0 + 64, X0:X0 + 64] = e_np[Y0:Y0 + 64, X0:X0 + 64]
+    assert np.array_equal(o_np, exp)
 
 
 @pytest.mark.parametrize("shape,roi,sr", [
