@@ -142,6 +142,11 @@ int bf_batch_run(bf_batch batch, void *stream);
 int bf_batch_run_graph(bf_batch batch, void *stream);
 /* Pixels the batch filters (sum of min(in,out) areas). */
 int64_t bf_batch_pixels(bf_batch batch);
+/* 1 when every job uses the canonical GAUSS-factory template at a radius
+ * with an unrolled kernel (3, 5, 7, 9, 15) and the batch runs those
+ * kernels (tile configs 0 and 1), 0 when it runs the tap-list kernel,
+ * < 0 on error.  Same output bits either way. */
+int bf_batch_unrolled(bf_batch batch);
 
 /* Autotuner (no reference counterpart).  bf_tune_radius times every tile
  * configuration for (radius, mode) on the current device -- a synthetic

@@ -91,6 +91,7 @@ def lib() -> ctypes.CDLL:
         "bf_batch_run": ([P, P], ctypes.c_int),
         "bf_batch_run_graph": ([P, P], ctypes.c_int),
         "bf_batch_pixels": ([P], I64),
+        "bf_batch_unrolled": ([P], ctypes.c_int),
         "bf_sur_param_count": ([], I64),
         "bf_sur_create": ([ctypes.c_uint64, ctypes.POINTER(P)], ctypes.c_int),
         "bf_sur_destroy": ([P], ctypes.c_int),
@@ -252,6 +253,7 @@ class Batch:
                                      ctypes.byref(h)), "bf_batch_create")
         self.handle = h
         self.pixels = int(lib().bf_batch_pixels(h))
+        self.unrolled = bool(_check(lib().bf_batch_unrolled(h), "bf_batch_unrolled"))
 
     def run(self, stream=None):
         _check(lib().bf_batch_run(self.handle, _stream(stream)), "bf_batch_run")

@@ -168,6 +168,7 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_b
     job->lut_key = ((lut->id & 0x7FFFF) << 12) | ((sep ? 0 : nrows) << 1) | (sep ? 1 : 0);
     job->tiles_x = (W + tc.tw - 1) / tc.tw;
     job->tile_begin = tile_begin;
+    job->radius = radius;
     if (pixels) *pixels = (int64_t)W * H;
     return kOk;
 }
@@ -187,22 +188,25 @@ static int tiles_of(const DevJob &j, int config)
 
 // Switch a job to the canonical per-q table of the unrolled kernels when
 // the template is the canonical ring, it holds the radius's taps and the
-// table fits next to a `config` tile.
-static bool to_ring_job(const Lut *L, int radius, int mode, int config, DevJob *job)
+// table fits next to a `config` tile with `halo`.
+bool ring_job(const Lut *L, int radius, int mode, int config, int halo, DevJob *job)
 {
     const int rr = ring_rows(radius, mode);
     if (!L->canonical || rr <= 0 || ntaps_for(radius, mode) > L->count || job->sep ||
-        lds_bytes(config, halo_class(radius), rr + 1) > max_lds_bytes())
+        halo < halo_class(radius) || lds_bytes(config, halo, rr + 1) > max_lds_bytes())
         return false;
     job->rowg = L->d_rowq;
     job->nrows = rr;
     job->lut_rows = rr + 1;
+    // LDS key as make_job forms it: with one weight per q the per-q table
+    // equals the per-weight one whenever their row counts agree
+    job->lut_key = ((L->id & 0x7FFFF) << 12) | (rr << 1);
     return true;
 }
 
 bool ring64_job(const Lut *L, int radius, int mode, DevJob *job)
 {
-    return ring64_available(radius) && to_ring_job(L, radius, mode, 0, job);
+    return ring64_available(radius) && ring_job(L, radius, mode, 0, halo_class(radius), job);
 }
 
 static bool views_overlap(const bf_image_view *a, const bf_image_view *b)
@@ -447,7 +451,7 @@ int bf_exact(const bf_image_view *in, const bf_image_view *out, bf_lut lut, int3
         DevJob job;
         int rc = make_job(in, out, L, radius, mode, rcfg, halo_class(radius), 0, &job, nullptr);
         if (rc) return rc;
-        if (to_ring_job(L, radius, mode, rcfg, &job))
+        if (ring_job(L, radius, mode, rcfg, halo_class(radius), &job))
             return launch_ring(job, radius, mode, tiles_of(job, rcfg), (hipStream_t)stream);
     }
     const int rows = fused_rows(lut_from_handle(lut), radius, mode);
@@ -603,6 +607,7 @@ struct Batch {
     int halo = 4;
     int max_rows = 0;
     int64_t pixels = 0;
+    bool ring = false;             // every job on the unrolled kernels (bf_ring_batch.hip)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = -1;
@@ -699,6 +704,16 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
             }
         }
     }
+    // Every job on the canonical template at a radius with an unrolled
+    // kernel: switch the whole batch to the per-q tables of those kernels.
+    bool ring = ring_batch_supported(cfg, halo);
+    if (ring) {
+        std::vector<DevJob> rj = dj;
+        for (int i = 0; i < njobs && ring; i++)
+            ring = ring_job(lut_from_handle(jobs[i].lut), jobs[i].radius, jobs[i].mode, cfg, halo,
+                            &rj[i]);
+        if (ring) dj.swap(rj);
+    }
     for (int i = 0; i < njobs; i++) max_lut_rows = std::max(max_lut_rows, dj[i].lut_rows);
     if (lds_bytes(cfg, halo, max_lut_rows) > max_lds_bytes()) {
         set_error("tile config does not fit the weight tables in LDS");
@@ -716,6 +731,7 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
     if (!B) return kOutOfMem;
     B->config = cfg;
     B->halo = halo;
+    B->ring = ring;
     B->max_rows = max_lut_rows;
     B->pixels = pixels;
     B->device = current_device();
@@ -744,6 +760,12 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
     return kOk;
 }
 
+int bf_batch_unrolled(bf_batch h)
+{
+    if (!h) return kNullPointer;
+    return reinterpret_cast<Batch *>(h)->ring ? 1 : 0;
+}
+
 int bf_batch_destroy(bf_batch h)
 {
     if (!h) return kNullPointer;
@@ -763,6 +785,9 @@ int bf_batch_run(bf_batch h, void *stream)
     hipStream_t s = (hipStream_t)stream;
     int rc = launch_reset_counter(B->d_counter, s);
     if (rc) return rc;
+    if (B->ring)
+        return launch_ring_batch(B->config, B->halo, B->d_jobs, (int)B->jobs.size(), B->ntiles,
+                                 B->max_rows, B->d_counter, s);
     return launch_exact(B->config, B->halo, B->jobs[0], B->d_jobs, (int)B->jobs.size(), B->ntiles,
                         B->max_rows, B->d_counter, s);
 }

@@ -82,6 +82,7 @@ struct alignas(16) DevJob {
     int32_t tile_begin;      // first global tile index of this job
     int32_t sep;             // 1: LDS holds E only, g*E formed per tap (large radii)
     int32_t lut_rows;        // 1 KB rows of the LDS table: nrows + 1, or 1 when sep
+    int32_t radius;
 };
 
 struct TileConfig {
@@ -106,6 +107,12 @@ size_t max_lds_bytes();
 //             grid-stride schedule.
 int launch_exact(int config, int halo, const DevJob &job0, const DevJob *jobs_dev,
                  int njobs, int ntiles, int max_lut_rows, int *counter, hipStream_t s);
+// The same launch for jobs that all use the canonical template at radii
+// with unrolled kernels (3, 5, 7, 9, 15; tables from ring_job), tile
+// configs 0 and 1, halo <= 16.
+bool ring_batch_supported(int config, int halo);
+int launch_ring_batch(int config, int halo, const DevJob *jobs_dev, int njobs, int ntiles,
+                      int max_lut_rows, int *counter, hipStream_t s);
 int launch_reset_counter(int *counter, hipStream_t s);
 
 // Build a DevJob for one view pair (validates; fills tiles_x for config;
@@ -135,6 +142,9 @@ int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_dev_
 // job made for config 0 to the canonical per-q table when they apply.
 bool ring64_available(int radius);
 bool ring64_job(const Lut *lut, int radius, int mode, DevJob *job);
+// Switch a job (made for `config` with `halo`) to the canonical per-q
+// table of the unrolled kernels; false (job unchanged) when they do not apply.
+bool ring_job(const Lut *lut, int radius, int mode, int config, int halo, DevJob *job);
 int ring64_tiles(const DevJob &job, int radius, int mode, const int32_t *list,
                  const int32_t *count, int ntiles_max, hipStream_t s);
 

@@ -1,6 +1,8 @@
 """Batched exact filter (bf_batch_*): one persistent launch over many images
 of uneven size, ROI and radius; bit-exact per image to the oracle, also
-when replayed from a hipGraph."""
+when replayed from a hipGraph.  Canonical-template batches at radii 3..15
+run the unrolled kernels (tile configs 0 and 1), others the tap-list
+kernel; both are covered."""
 import numpy as np
 import pytest
 
@@ -54,6 +56,7 @@ class BatchCase:
 
 def test_batch_small_c2_subset(device):
     bc = BatchCase(wl.c2(24))
+    assert bc.batch.unrolled
     bc.batch.run()
     bc.check()
 
@@ -65,6 +68,7 @@ def test_batch_mixed_modes_and_table_modes(device):
     items = items + [big, items[0]]
     modes = [oc.MODE_REF, oc.MODE_STD] * 3 + [oc.MODE_STD, oc.MODE_STD]
     bc = BatchCase(items, modes)
+    assert not bc.batch.unrolled          # r = 19 has no unrolled kernel
     bc.batch.run()
     bc.check()
 
@@ -73,6 +77,7 @@ def test_every_config_identical(device):
     items = wl.c2(10)
     for cfg, tw, th, nt in oc.tune_configs():
         bc = BatchCase(items, config=cfg)
+        assert bc.batch.unrolled == (cfg in (0, 1))
         bc.batch.run()
         bc.check()
 
