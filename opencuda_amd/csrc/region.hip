@@ -196,14 +196,13 @@ __device__ __forceinline__ bool cr_fg(const CrArgs &a, uint32_t v)
     return (int)v >= a.min_gray && (int)v <= a.max_gray;
 }
 
+// Areas start at zero (every label is written by the tile pass).
 __global__ void cr_init_kernel(CrArgs a)
 {
     const int64_t n = (int64_t)a.W * a.H;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        a.label[i] = (int)i;
+         i += (int64_t)gridDim.x * blockDim.x)
         a.area[i] = 0;
-    }
 }
 
 // Tile pass: each 64 x 16 tile labels its own pixels in LDS with the same
