@@ -243,10 +243,12 @@ constexpr int ring_min_offset()
     return m;
 }
 
-// Taps scheduled together in the unrolled kernels (STD has 4 fetches per
-// tap, so it keeps fewer taps in flight).
-template <int MODE>
-constexpr int ring_group() { return MODE == kModeRef ? 4 : 2; }
+// Taps scheduled together between fences in the unrolled kernels.
+// (Measured on MI355X, c3 image: REF is flat from 3 to 8 taps -- its VALU
+// is ~94% busy -- while STD gains 8-14% going from 2 taps to 12 at r = 15
+// and 24 at r = 3..9; at r = 15, 16 and 24 taps measured slower than 12.)
+template <int MODE, int R>
+constexpr int ring_group() { return MODE == kModeRef ? 4 : (R >= 13 ? 12 : 24); }
 
 template <class F, int... I>
 __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
@@ -309,7 +311,7 @@ __device__ __forceinline__ void tile_ring(const DevJob &job, int X0, int Y0, int
         // the window first and spills them.  The empty asm reads and writes
         // the accumulators and clobbers memory, so neither the adds nor the
         // LDS reads move across it; it emits no instruction.
-        if constexpr (decltype(ci)::value % ring_group<MODE>() == ring_group<MODE>() - 1)
+        if constexpr (decltype(ci)::value % ring_group<MODE, R>() == ring_group<MODE, R>() - 1)
             asm volatile("" : "+v"(t0), "+v"(t1), "+v"(t2), "+v"(t3), "+v"(s0), "+v"(s1),
                          "+v"(s2), "+v"(s3)::"memory");
     });
