@@ -1,9 +1,13 @@
 """Throughput of the batched and mixed workloads on the current GPU.
 
+    python -m opencuda_amd.measure c1 [--steps 20]    # one large ROI, r = 7, both modes
     python -m opencuda_amd.measure c2 [--steps 5]     # batched persistent launch
     python -m opencuda_amd.measure c4 [--steps 3]     # exact batch + surrogate, one graph
     python -m opencuda_amd.measure c3-surrogate       # the surrogate on the 8192^2 image
     python -m opencuda_amd.measure chain              # LocalCluster -> filter -> ConnectRegion
+
+c1 (4096^2 image, ROI 3800x3600 at (100, 200), r = 7): the exact filter
+in REF and STD mode, Mpix/s and algorithmic TFLOP/s (4T+1 FLOP/px).
 
 c2 (256 images, 256..2048 per side, ROI 50..100 %, r in {3,5,9}, REF mode):
 one bf_batch launch (eager and replayed from a captured graph) against the
@@ -52,6 +56,28 @@ def _time(fn, steps, stream):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / steps
+
+
+def c1(steps: int = 20):
+    import torch
+    (it,) = wl.c1()
+    tin = torch.from_numpy(it.image()).cuda()
+    x1, y1, x2, y2 = it.roi
+    tout = torch.zeros((y2 - y1, x2 - x1), dtype=torch.uint8, device="cuda")
+    lut = Lut.from_sigmas(it.sigma_space, it.sigma_range)
+    stream = torch.cuda.current_stream()
+    px = (x2 - x1) * (y2 - y1)
+    res = {"workload": f"c1: {x2 - x1}x{y2 - y1} ROI of a 4096^2 image, r={it.radius}",
+           "pixels": px}
+    for name, mode in (("ref", MODE_REF), ("std", MODE_STD)):
+        f = lambda: exact(view(tin, it.roi), view(tout), lut, it.radius, mode, stream=stream)
+        _time(f, 8, stream)                     # past the clock ramp
+        ms = _time(f, steps, stream)
+        taps = (2 * it.radius + 1) ** 2 + (1 if mode == MODE_REF else 0)
+        res[name] = {"ms": round(ms, 4), "mpix_s": round(px / ms / 1e3, 1),
+                     "tflops": round(px * (4 * taps + 1) / ms / 1e9, 2)}
+        _log(f"c1 {name}: {ms:.3f} ms")
+    return res
 
 
 def c2(steps: int = 5, mode: int = MODE_REF, tune_table: str | None = None):
@@ -205,7 +231,7 @@ def chain(steps: int = 5, size: int = 4096):
 
 def _main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("workload", choices=["c2", "c4", "c3-surrogate", "chain"])
+    ap.add_argument("workload", choices=["c1", "c2", "c4", "c3-surrogate", "chain"])
     ap.add_argument("--steps", type=int, default=0)
     ap.add_argument("--count", type=int, default=512, help="c4: number of images")
     ap.add_argument("--mode", choices=["ref", "std"], default=None)
@@ -214,7 +240,9 @@ def _main() -> int:
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
-    if args.workload == "c3-surrogate":
+    if args.workload == "c1":
+        res = c1(args.steps or 20)
+    elif args.workload == "c3-surrogate":
         res = c3_surrogate(args.steps or 5)
     elif args.workload == "chain":
         res = chain(args.steps or 5)
