@@ -206,8 +206,9 @@ def main() -> int:
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}, "
-              f"WORLD_SIZE {world}); the filter shards by row bands with no exchange, "
-              "see DESIGN.md. Run with --gpus 1.", file=sys.stderr)
+              f"WORLD_SIZE {world}): this build runs on a single GPU; the filter "
+              "would shard by row bands (read-only halo, no exchange). Run with --gpus 1.",
+              file=sys.stderr)
         return 2
 
     import numpy as np
