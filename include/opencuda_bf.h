@@ -198,18 +198,23 @@ int bf_sur_set_params(bf_sur sur, const float *params);
 /* One training step on a batch of n u8 crops (h x w, dense, device):
  * forward, MSE loss against target/255, backward, Adam (lr, beta1, beta2,
  * eps; bias-corrected, step counter kept in the net).  sigma_range: n
- * floats on the device.  loss_out: device float (mean loss), may be NULL. */
+ * floats on the device.  The loss leaves out a 6-pixel band along each
+ * crop side, except the sides that lie on their image's edge: edges (n
+ * bytes on the device, or NULL = no such side) has bit 0 top, 1 bottom,
+ * 2 left, 3 right.  loss_out: device float (mean loss), may be NULL. */
 int bf_sur_train_step(bf_sur sur, const unsigned char *x, const unsigned char *y,
-                      const float *sigma_range, int32_t n, int32_t h, int32_t w,
+                      const float *sigma_range, const unsigned char *edges,
+                      int32_t n, int32_t h, int32_t w,
                       float lr, float beta1, float beta2, float eps,
                       float *loss_out, void *stream);
 
 /* Forward only, FP32 output (n x h x w, device) of the same crops: the
  * gradient-check hook.  grads (optional, device, param_count floats):
- * receives d(loss)/d(params) of the MSE loss against y (y may be NULL
- * when grads is NULL). */
+ * receives d(loss)/d(params) of the loss bf_sur_train_step minimises
+ * against y (y may be NULL when grads is NULL; edges as there). */
 int bf_sur_forward_backward(bf_sur sur, const unsigned char *x,
                             const unsigned char *y, const float *sigma_range,
+                            const unsigned char *edges,
                             int32_t n, int32_t h, int32_t w, float *pred,
                             float *grads, float *loss_out, void *stream);
 

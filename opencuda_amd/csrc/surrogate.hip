@@ -395,10 +395,16 @@ __global__ void reduce_partials(const float *partial, int nparts, int psz, float
 
 // ------------------------------------------------------------- loss, Adam
 
-// pred vs target/255 on pixels at least MARGIN from the crop border:
-// g6 = 2 (pred - y) / count (bf16), loss += (pred - y)^2 / count.
-__global__ void loss_kernel(const float *pred, const uint8_t *target, int N, int H, int W,
-                            __bf16 *g6, float *loss)
+// pred vs target/255 on pixels at least MARGIN from the crop border, except
+// along the sides that lie on the image's own edge (edges[n]: bit 0 top,
+// 1 bottom, 2 left, 3 right; NULL = none): there the target is the exact
+// filter's clamp-to-edge result, which the network can see (its inputs are
+// zero past the edge), so those pixels are trained too.
+// g6 = 2 (pred - y) / count (bf16), loss += (pred - y)^2 / count, with
+// count = N (H - 2 MARGIN) (W - 2 MARGIN) whatever the edges (the step size
+// does not depend on a batch's mix of edge crops).
+__global__ void loss_kernel(const float *pred, const uint8_t *target, const uint8_t *edges,
+                            int N, int H, int W, __bf16 *g6, float *loss)
 {
     __shared__ float red[256];
     const int64_t total = (int64_t)N * H * W;
@@ -408,8 +414,11 @@ __global__ void loss_kernel(const float *pred, const uint8_t *target, int N, int
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(i % W), y = (int)((i / W) % H);
+        const int e = edges ? edges[i / ((int64_t)H * W)] : 0;
+        const int mt = (e & 1) ? 0 : MARGIN, mb = (e & 2) ? 0 : MARGIN;
+        const int ml = (e & 4) ? 0 : MARGIN, mr = (e & 8) ? 0 : MARGIN;
         float g = 0.0f;
-        if (x >= MARGIN && x < W - MARGIN && y >= MARGIN && y < H - MARGIN) {
+        if (x >= ml && x < W - mr && y >= mt && y < H - mb) {
             const float d = pred[i] - (float)target[i] / 255.0f;
             acc += d * d * inv;
             g = 2.0f * d * inv;
@@ -1045,14 +1054,14 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
     return kOk;
 }
 
-// Backward pass after forward(): gradients of the mean interior MSE into
-// net->grad.  loss (device float) is accumulated (caller zeroes it).
-int backward(Net *net, const uint8_t *img, const uint8_t *target, const float *sigma, int N,
-             int H, int W, float *loss, hipStream_t s)
+// Backward pass after forward(): gradients of the masked MSE (loss_kernel)
+// into net->grad.  loss (device float) is accumulated (caller zeroes it).
+int backward(Net *net, const uint8_t *img, const uint8_t *target, const uint8_t *edges,
+             const float *sigma, int N, int H, int W, float *loss, hipStream_t s)
 {
     const int64_t total = (int64_t)N * H * W;
     const int lb = (int)std::min<int64_t>(4096, (total + 255) / 256);
-    hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, s, net->pred, target, N, H, W,
+    hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, s, net->pred, target, edges, N, H, W,
                        net->g6, loss);
     BF_HIP(hipGetLastError());
     int rc;
@@ -1221,8 +1230,8 @@ static int check_batch(const void *x, const float *sigma, int n, int h, int w)
 }
 
 int bf_sur_forward_backward(bf_sur hnd, const unsigned char *x, const unsigned char *y,
-                            const float *sigma, int32_t n, int32_t h, int32_t w, float *pred,
-                            float *grads, float *loss_out, void *stream)
+                            const float *sigma, const unsigned char *edges, int32_t n, int32_t h,
+                            int32_t w, float *pred, float *grads, float *loss_out, void *stream)
 {
     if (!hnd) return kNullPointer;
     int rc = check_batch(x, sigma, n, h, w);
@@ -1243,15 +1252,15 @@ int bf_sur_forward_backward(bf_sur hnd, const unsigned char *x, const unsigned c
         BF_HIP(hipGetLastError());
     }
     if (y) {
-        if ((rc = backward(net, x, y, sigma, n, h, w, loss_out, s))) return rc;
+        if ((rc = backward(net, x, y, edges, sigma, n, h, w, loss_out, s))) return rc;
         if (grads) BF_HIP(hipMemcpyAsync(grads, net->grad, kParams * 4, hipMemcpyDeviceToDevice, s));
     }
     return kOk;
 }
 
 int bf_sur_train_step(bf_sur hnd, const unsigned char *x, const unsigned char *y, const float *sigma,
-                      int32_t n, int32_t h, int32_t w, float lr, float beta1, float beta2, float eps,
-                      float *loss_out, void *stream)
+                      const unsigned char *edges, int32_t n, int32_t h, int32_t w, float lr,
+                      float beta1, float beta2, float eps, float *loss_out, void *stream)
 {
     if (!hnd || !y) return kNullPointer;
     int rc = check_batch(x, sigma, n, h, w);
@@ -1265,7 +1274,7 @@ int bf_sur_train_step(bf_sur hnd, const unsigned char *x, const unsigned char *y
         hipLaunchKernelGGL(zero_floats, dim3(1), dim3(64), 0, s, loss_out, 1);
         BF_HIP(hipGetLastError());
     }
-    if ((rc = backward(net, x, y, sigma, n, h, w, loss_out, s))) return rc;
+    if ((rc = backward(net, x, y, edges, sigma, n, h, w, loss_out, s))) return rc;
     net->step++;
     const float c1 = 1.0f - std::pow(beta1, (float)net->step);
     const float c2 = 1.0f - std::pow(beta2, (float)net->step);

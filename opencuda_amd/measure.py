@@ -142,10 +142,11 @@ def c2(steps: int = 5, mode: int = MODE_REF, tune_table: str | None = None):
     }
 
 
-def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD, streams: int = 4):
+def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD, streams: int = 4,
+       weights: str | None = None):
     import torch
     from .mixed import measure
-    out, mx = measure(count, steps, mode, log=_log, streams=streams)
+    out, mx = measure(count, steps, mode, log=_log, streams=streams, weights=weights)
     stream = torch.cuda.current_stream()
     jobs = [Job(view(mx.tin[i], it.roi), view(mx.tout[i]),
                 mx.luts[(it.sigma_space, it.sigma_range)].handle, it.radius, mode)
@@ -165,7 +166,7 @@ def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD, streams: int = 4)
     return out
 
 
-def c3_surrogate(steps: int = 5):
+def c3_surrogate(steps: int = 5, weights: str | None = None):
     import torch
     from .surrogate import DEFAULT_WEIGHTS, Surrogate
     (it,) = wl.c3()
@@ -174,7 +175,7 @@ def c3_surrogate(steps: int = 5):
     tout = torch.zeros_like(tin)
     lut = Lut.from_sigmas(it.sigma_space, it.sigma_range)
     net = Surrogate(seed=1)
-    net.load(DEFAULT_WEIGHTS)
+    net.load(weights or DEFAULT_WEIGHTS)
     stream = torch.cuda.current_stream()
     vin, vout = view(tin), view(tout)
     for _ in range(2):
@@ -237,13 +238,14 @@ def _main() -> int:
     ap.add_argument("--mode", choices=["ref", "std"], default=None)
     ap.add_argument("--tune", default=None, help="c2: also time with this autotuner table")
     ap.add_argument("--streams", type=int, default=4, help="c4: streams for the surrogate images")
+    ap.add_argument("--weights", default=None, help="c4, c3-surrogate: surrogate weights (.npz)")
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
     if args.workload == "c1":
         res = c1(args.steps or 20)
     elif args.workload == "c3-surrogate":
-        res = c3_surrogate(args.steps or 5)
+        res = c3_surrogate(args.steps or 5, args.weights)
     elif args.workload == "chain":
         res = chain(args.steps or 5)
     elif args.workload == "c2":
@@ -251,7 +253,7 @@ def _main() -> int:
         res = c2(args.steps or 5, mode, args.tune)
     else:
         mode = MODE_REF if args.mode == "ref" else MODE_STD
-        res = c4(args.count, args.steps or 3, mode, args.streams)
+        res = c4(args.count, args.steps or 3, mode, args.streams, args.weights)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -128,10 +128,10 @@ class Mixed:
         return ref
 
 
-def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4):
+def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None):
     import torch
     net = Surrogate(seed=1)
-    net.load(DEFAULT_WEIGHTS)
+    net.load(weights or DEFAULT_WEIGHTS)
     mx = Mixed(wl.c4(count), net, mode=mode, streams=streams, log=log)
     if log:
         log(f"{len(mx.ex_idx)} exact images, {len(mx.sur_idx)} surrogate images, "

@@ -59,16 +59,19 @@ class Surrogate:
 
     # ---- training
     def train_step(self, x, y, sigma, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8,
-                   loss=None, stream=None) -> None:
-        """x, y: uint8 device tensors (n, h, w); sigma: float32 device (n)."""
+                   loss=None, stream=None, edges=None) -> None:
+        """x, y: uint8 device tensors (n, h, w); sigma: float32 device (n);
+        edges: optional uint8 device (n), the crop sides on an image edge
+        (bit 0 top, 1 bottom, 2 left, 3 right), trained without margin."""
         n, h, w = x.shape
         lp = ctypes.c_void_p(loss.data_ptr()) if loss is not None else None
+        ep = ctypes.c_void_p(edges.data_ptr()) if edges is not None else None
         _check(lib().bf_sur_train_step(self.handle, ctypes.c_void_p(x.data_ptr()),
                                        ctypes.c_void_p(y.data_ptr()),
-                                       ctypes.c_void_p(sigma.data_ptr()), n, h, w, lr, beta1,
+                                       ctypes.c_void_p(sigma.data_ptr()), ep, n, h, w, lr, beta1,
                                        beta2, eps, lp, _stream(stream)), "bf_sur_train_step")
 
-    def forward_backward(self, x, y, sigma, stream=None):
+    def forward_backward(self, x, y, sigma, stream=None, edges=None):
         """Returns (pred float32 (n,h,w), grads float32 (nparams), loss) tensors."""
         import torch
         n, h, w = x.shape
@@ -78,7 +81,9 @@ class Surrogate:
         _check(lib().bf_sur_forward_backward(
             self.handle, ctypes.c_void_p(x.data_ptr()),
             ctypes.c_void_p(y.data_ptr()) if y is not None else None,
-            ctypes.c_void_p(sigma.data_ptr()), n, h, w, ctypes.c_void_p(pred.data_ptr()),
+            ctypes.c_void_p(sigma.data_ptr()),
+            ctypes.c_void_p(edges.data_ptr()) if edges is not None else None,
+            n, h, w, ctypes.c_void_p(pred.data_ptr()),
             ctypes.c_void_p(grads.data_ptr()) if grads is not None else None,
             ctypes.c_void_p(loss.data_ptr()), _stream(stream)), "bf_sur_forward_backward")
         return pred, grads, loss
@@ -162,23 +167,38 @@ class Dataset:
         torch.cuda.synchronize()
         self.luts = luts
 
-    def batch(self, rng: np.random.Generator, n: int = 64, crop: int = 256):
+    def batch(self, rng: np.random.Generator, n: int = 64, crop: int = 256,
+              edge_frac: float = 0.0):
+        """n random crops; with probability edge_frac a crop's row (and,
+        independently, its column) offset is snapped to the image's first or
+        last position, so image edges and corners are trained.  Returns
+        x, y, sigma and the edge bits of each crop (bit 0 top, 1 bottom,
+        2 left, 3 right)."""
         import torch
         idx = torch.from_numpy(rng.integers(0, self.x.shape[0], n)).cuda()
-        oy = rng.integers(0, self.size - crop + 1, n)
-        ox = rng.integers(0, self.size - crop + 1, n)
+        last = self.size - crop
+        oy = rng.integers(0, last + 1, n)
+        ox = rng.integers(0, last + 1, n)
+        if edge_frac > 0:
+            sy = rng.random(n) < edge_frac
+            sx = rng.random(n) < edge_frac
+            oy = np.where(sy, np.where(rng.random(n) < 0.5, 0, last), oy)
+            ox = np.where(sx, np.where(rng.random(n) < 0.5, 0, last), ox)
+        edges = ((oy == 0) * 1 + (oy == last) * 2 + (ox == 0) * 4 + (ox == last) * 8).astype(np.uint8)
         ar = torch.arange(crop, device="cuda")
         rows = (torch.from_numpy(oy).cuda()[:, None] + ar[None, :])[:, :, None]
         cols = (torch.from_numpy(ox).cuda()[:, None] + ar[None, :])[:, None, :]
         ii = idx[:, None, None]
         return (self.x[ii, rows, cols].contiguous(), self.y[ii, rows, cols].contiguous(),
-                self.sigma[idx].contiguous())
+                self.sigma[idx].contiguous(), torch.from_numpy(edges).cuda())
 
 
 def train(net: Surrogate, data: Dataset, steps: int = 1500, batch: int = 64, crop: int = 256,
-          lr: float = 1e-3, seed: int = 7, log=None, log_every: int = 100, cosine: bool = False):
+          lr: float = 1e-3, seed: int = 7, log=None, log_every: int = 100, cosine: bool = False,
+          edge_frac: float = 0.0):
     """Adam on random crops (learning rate `lr`, or `lr` decayed to 0 along
-    a half cosine when `cosine`); returns the list of (step, loss)."""
+    a half cosine when `cosine`; `edge_frac` as in Dataset.batch); returns
+    the list of (step, loss)."""
     import math
 
     import torch
@@ -187,9 +207,9 @@ def train(net: Surrogate, data: Dataset, steps: int = 1500, batch: int = 64, cro
     hist = []
     t0 = time.time()
     for s in range(1, steps + 1):
-        x, y, sg = data.batch(rng, batch, crop)
+        x, y, sg, edges = data.batch(rng, batch, crop, edge_frac)
         lr_s = lr * 0.5 * (1.0 + math.cos(math.pi * (s - 1) / steps)) if cosine else lr
-        net.train_step(x, y, sg, lr=lr_s, loss=loss)
+        net.train_step(x, y, sg, lr=lr_s, loss=loss, edges=edges if edge_frac > 0 else None)
         if s % log_every == 0 or s == 1 or s == steps:
             v = float(loss.item())
             hist.append((s, v))
@@ -238,6 +258,8 @@ def _main() -> int:
     ap.add_argument("--eval", action="store_true", help="report accuracy on the c3 image")
     ap.add_argument("--init", default=None, help="start from these weights (.npz)")
     ap.add_argument("--log-every", type=int, default=100)
+    ap.add_argument("--edge-frac", type=float, default=0.0,
+                    help="fraction of crop offsets snapped to an image edge (trained unmasked)")
     args = ap.parse_args()
 
     def log(m):
@@ -251,11 +273,11 @@ def _main() -> int:
         init_steps = int(net.load(args.init).get("steps", 0))
         log(f"starting from {args.init} ({init_steps} steps)")
     hist = train(net, data, args.steps, args.batch, args.crop, args.lr, log=log,
-                 cosine=args.cosine, log_every=args.log_every)
+                 cosine=args.cosine, log_every=args.log_every, edge_frac=args.edge_frac)
     net.save(args.out, history=np.array(hist, np.float64), mode=mode, radius=15,
              sigma_space=10.0, images=args.images, steps=init_steps + args.steps, batch=args.batch,
              crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine),
-             init_steps=init_steps)
+             init_steps=init_steps, edge_frac=args.edge_frac)
     log(f"saved {args.out}")
     if args.eval:
         from . import workloads

@@ -41,8 +41,11 @@ def _unpack(p):
     return out
 
 
-def _reference(params, x_u8, y_u8, sigma, margin=6):
-    """float64 CPU network with the GPU's BF16 rounding points."""
+def _reference(params, x_u8, y_u8, sigma, margin=6, edges=None):
+    """float64 CPU network with the GPU's BF16 rounding points; the loss
+    leaves out `margin` pixels along each crop side except the sides that
+    `edges[n]` marks as image edges (bit 0 top, 1 bottom, 2 left, 3 right),
+    normalised by the interior pixel count."""
     import torch
     import torch.nn.functional as F
 
@@ -74,8 +77,14 @@ def _reference(params, x_u8, y_u8, sigma, margin=6):
     out = F.conv2d(a, RoundSTE.apply(ws[5]), bs[5], padding=1)[:, 0]
     pred = x / 255.0 + out
     y = torch.tensor(y_u8, dtype=torch.float64) / 255.0
-    d = (pred - y)[:, margin:h - margin, margin:w - margin]
-    loss = (d * d).mean()
+    mask = torch.zeros((n, h, w), dtype=torch.float64)
+    for i in range(n):
+        e = 0 if edges is None else int(edges[i])
+        t, b = (0 if e & 1 else margin), (0 if e & 2 else margin)
+        le, r = (0 if e & 4 else margin), (0 if e & 8 else margin)
+        mask[i, t:h - b, le:w - r] = 1.0
+    d = (pred - y) * mask
+    loss = (d * d).sum() / (n * (h - 2 * margin) * (w - 2 * margin))
     loss.backward()
     grads = []
     for (cin, cout), wp, bp in zip(LAYERS, ws, bs):
@@ -88,7 +97,8 @@ def test_param_count(device):
     assert oc.lib().bf_sur_param_count() == 37889
 
 
-def test_forward_and_gradients_match_cpu_float64(device):
+@pytest.mark.parametrize("edges", [None, (1 | 4, 2 | 8)])
+def test_forward_and_gradients_match_cpu_float64(device, edges):
     import torch
     rng = np.random.default_rng(0)
     net = sg.Surrogate(seed=3)
@@ -101,10 +111,11 @@ def test_forward_and_gradients_match_cpu_float64(device):
     sigma = np.array([20.0, 37.5], np.float32)
     tx, ty = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
     ts = torch.from_numpy(sigma).cuda()
-    pred, grads, loss = net.forward_backward(tx, ty, ts)
+    te = torch.tensor(edges, dtype=torch.uint8).cuda() if edges is not None else None
+    pred, grads, loss = net.forward_backward(tx, ty, ts, edges=te)
     torch.cuda.synchronize()
     pred, grads, loss = pred.cpu().numpy(), grads.cpu().numpy(), float(loss.item())
-    rpred, rgrads, rloss = _reference(p, x, y, sigma)
+    rpred, rgrads, rloss = _reference(p, x, y, sigma, edges=edges)
     assert np.abs(pred - rpred).max() < 2e-3
     assert abs(loss - rloss) / rloss < 1e-2
     o = 0
