@@ -247,7 +247,6 @@ __device__ __forceinline__ void check_sample(int tile, int cell, uint32_t salt, 
 __device__ __forceinline__ void check_verdict(const CheckArgs &a, int tile, bool bad)
 {
     if (__any(bad) && (threadIdx.x & 63) == 0) {
-        a.flags[tile] = 1;
         const int slot = atomicAdd(a.count, 1);
         a.list[slot] = tile;
         if (a.accum) atomicAdd(a.accum, 1);

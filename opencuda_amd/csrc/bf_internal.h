@@ -154,7 +154,6 @@ struct CheckArgs {
     const float *rowg;           // weights of the tap rows (run-time tap list)
     int tiles_x, tiles_y;
     int tol;
-    int32_t *flags;              // tiles_x * tiles_y
     int32_t *list;               // fallback tile ids
     int32_t *count;
     int32_t *accum;              // optional running total of fallback tiles

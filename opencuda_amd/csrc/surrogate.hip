@@ -911,9 +911,8 @@ struct Net {
     float *partial = nullptr;
     int partial_blocks = 0;
     // fallback workspace
-    int32_t *flags = nullptr, *list = nullptr, *count = nullptr;
+    int32_t *list = nullptr, *count = nullptr;
     int fb_tiles = 0;
-    int32_t *host_count = nullptr;
 };
 
 Net *net_of(bf_sur h) { return reinterpret_cast<Net *>(h); }
@@ -1116,10 +1115,8 @@ void free_net(Net *net)
     hipFree(net->g6);
     hipFree(net->pred);
     hipFree(net->partial);
-    hipFree(net->flags);
     hipFree(net->list);
     hipFree(net->count);
-    if (net->host_count) hipHostFree(net->host_count);
     delete net;
 }
 
@@ -1322,16 +1319,13 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     if ((rc = infer_view(net, in, out, sigma_range, s, &W, &H))) return rc;
     const int tx = (W + 63) / 64, ty = (H + 63) / 64, nt = tx * ty;
     if (nt > net->fb_tiles) {
-        hipFree(net->flags);
         hipFree(net->list);
-        net->flags = net->list = nullptr;
+        net->list = nullptr;
         net->fb_tiles = 0;
-        BF_HIP(hipMalloc(&net->flags, nt * 4));
         BF_HIP(hipMalloc(&net->list, nt * 4));
         net->fb_tiles = nt;
     }
     if (!net->count) BF_HIP(hipMalloc(&net->count, 4));
-    hipLaunchKernelGGL(zero_ints, dim3((nt + 255) / 256), dim3(256), 0, s, net->flags, nt);
     hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, net->count, 1);
     BF_HIP(hipGetLastError());
     // The canonical template at r = 9 or 15 runs the unrolled 64x64-tile
@@ -1344,7 +1338,6 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     ca.tiles_x = tx;
     ca.tiles_y = ty;
     ca.tol = tol_lsb;
-    ca.flags = net->flags;
     ca.list = net->list;
     ca.count = net->count;
     ca.accum = accum;
