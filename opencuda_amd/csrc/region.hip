@@ -319,48 +319,68 @@ __global__ void cr_compress_kernel(CrArgs a)
         a.label[i] = cr_find(a.label, (int)i);
 }
 
-// Areas per root over runs of up to 64 rows of one column, rows below
-// a.covered only.  A run that ends at a label change is added by its lane;
-// the runs that reach the segment's end are combined across the wave first
-// (one atomic per distinct label), since inside a large region every lane
-// ends on the same root.
-constexpr int kAreaRows = 64;
-
+// Areas per root over the rows below a.covered: one wave per row walks it
+// in 64-pixel chunks.  Inside a chunk the lane that starts a run of equal
+// labels adds the run; the run that reaches the chunk's end is carried,
+// with a wave-uniform label and count, into the next chunk.  A row's first
+// long run (>= 32 pixels) names its "hot" label, whose later runs are
+// summed in the wave and added once at the row's end: in a large region
+// broken by many small ones (here 1.26M runs on a 4096^2 image, half of them
+// one region's) every run would otherwise be an atomic on the same
+// counter, and those serialise.
 __global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
 {
-    const int x = blockIdx.x * 256 + threadIdx.x;
-    const int y0 = blockIdx.y * kAreaRows;
-    const bool active = x < a.W && y0 < a.covered;
-    int cur = -1, run = 0;
-    if (active) {
-        const int y1 = min(y0 + kAreaRows, a.covered);
-        cur = a.label[y0 * a.W + x];
-        run = 1;
-        for (int y = y0 + 1; y < y1; y++) {
-            const int l = a.label[y * a.W + x];
-            if (l != cur) {
-                atomicAdd(a.area + cur, run);
-                cur = l;
-                run = 1;
-            } else {
-                run++;
-            }
+    const int lane = threadIdx.x & 63;
+    const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (y >= a.covered) return;                     // whole waves
+    const int *row = a.label + (int64_t)y * a.W;
+    int cur = -1, cnt = 0;                          // carried run
+    int hot = -1, hot_cnt = 0;                      // the row's hot label and its sum
+    // a finished run of `lab` (wave-uniform): to the hot sum or one atomic
+    auto flush = [&](int lab, int c) {
+        if (lab < 0 || c <= 0) return;
+        if (lab == hot) {
+            hot_cnt += c;
+        } else if (hot < 0 && c >= 32) {
+            hot = lab;
+            hot_cnt = c;
+        } else if (lane == 0) {
+            atomicAdd(a.area + lab, c);
+        }
+    };
+    for (int x0 = 0; x0 < a.W; x0 += 64) {
+        const int n = min(64, a.W - x0);            // pixels in this chunk
+        const int l = lane < n ? row[x0 + lane] : -1;
+        int prev = __shfl_up(l, 1);
+        if (lane == 0) prev = cur;
+        const bool start = lane < n && l != prev;
+        const uint64_t starts = __ballot(start);
+        // runs that start and end inside the chunk: the hot label's are
+        // summed across the wave, the others added by their start lane
+        int len = 0;
+        if (start) {
+            const uint64_t later = lane == 63 ? 0ull : (starts >> (lane + 1)) << (lane + 1);
+            if (later) len = __ffsll((unsigned long long)later) - 1 - lane;
+        }
+        const bool to_hot = len > 0 && l == hot;
+        if (len > 0 && !to_hot) atomicAdd(a.area + l, len);
+        if (__ballot(to_hot)) {
+            int sum = to_hot ? len : 0;
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+            hot_cnt += sum;
+        }
+        const int first = starts ? __ffsll((unsigned long long)starts) - 1 : n;
+        cnt += first;                               // the carried run goes on to `first`
+        if (starts) {
+            flush(cur, cnt);
+            const int last = 63 - __clzll((long long)starts);
+            cur = __shfl(l, last);
+            cnt = n - last;
         }
     }
-    // combine the final runs of the wave, one distinct label at a time
-    const int lane = threadIdx.x & 63;
-    uint64_t todo = __ballot(active);
-    while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const int lab = __shfl(cur, leader);
-        const bool mine = active && cur == lab;
-        const uint64_t group = __ballot(mine);
-        int sum = mine ? run : 0;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-        if (lane == leader) atomicAdd(a.area + lab, sum);
-        todo &= ~group;
-    }
+    flush(cur, cnt);
+    if (lane == 0 && hot >= 0) atomicAdd(a.area + hot, hot_cnt);
 }
 
 __device__ __forceinline__ bool cr_kept_root(const CrArgs &a, int i)
@@ -541,8 +561,7 @@ int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t
     hipLaunchKernelGGL(cr_union_kernel, g2, dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_compress_kernel, dim3(flat), dim3(256), 0, s, a);
     if (a.covered > 0)
-        hipLaunchKernelGGL(cr_area_kernel, dim3((W + 255) / 256, (a.covered + kAreaRows - 1) / kAreaRows),
-                           dim3(256), 0, s, a);
+        hipLaunchKernelGGL(cr_area_kernel, dim3((a.covered + 3) / 4), dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_rowcount_kernel, dim3(H), dim3(64), 0, s, a);
     hipLaunchKernelGGL(cr_scan_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(cr_number_kernel, dim3(H), dim3(64), 0, s, a);
