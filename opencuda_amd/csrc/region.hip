@@ -251,64 +251,93 @@ __device__ void l_unite(int *L, int a, int b)
 
 constexpr int kCrLocalThreads = 256;    // 512 and 1024 measured no faster
 
+// A tile row is one wave (kCrTw = 64): its foreground runs are linked to
+// their first pixel with a ballot (no atomics), and a run is united with
+// the row above once per run it touches there: through N at the first of
+// its pixels under each above segment, through NW / NE at its ends when N
+// is background there (a diagonal neighbour next to a foreground N pixel is
+// in that pixel's run already).
+static_assert(kCrTw == 64, "one wave per tile row");
+
 __global__ void __launch_bounds__(kCrLocalThreads) cr_local_kernel(CrArgs a)
 {
+    constexpr int kWaves = kCrLocalThreads / 64;
     __shared__ int loc[kCrTw * kCrTh];
-    __shared__ uint8_t fg[kCrTw * kCrTh];
+    __shared__ uint64_t fgrow[kCrTh];            // foreground bits of each tile row
     const int x0 = blockIdx.x * kCrTw, y0 = blockIdx.y * kCrTh;
-    constexpr int kPer = kCrTw * kCrTh / kCrLocalThreads;
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int i = (int)threadIdx.x + kCrLocalThreads * k;
-        const int gx = x0 + (i & (kCrTw - 1)), gy = y0 + i / kCrTw;
-        fg[i] = gx < a.W && gy < a.H && cr_fg(a, a.in[(int64_t)gy * a.pitch + gx]);
-        loc[i] = i;
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int gx = x0 + lane;
+    const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;   // lanes 0..lane
+    for (int ly = wave; ly < kCrTh; ly += kWaves) {
+        const int gy = y0 + ly;
+        const bool f = gx < a.W && gy < a.H && cr_fg(a, a.in[(int64_t)gy * a.pitch + gx]);
+        const uint64_t m = __ballot(f);
+        const uint64_t starts = m & ~(m << 1);   // foreground with background on the left
+        const int s = f ? 63 - __clzll((long long)(starts & upto)) : lane;
+        loc[ly * kCrTw + lane] = ly * kCrTw + s;
+        if (lane == 0) fgrow[ly] = m;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int i = (int)threadIdx.x + kCrLocalThreads * k;
-        if (!fg[i]) continue;
-        const int lx = i & (kCrTw - 1), ly = i / kCrTw;
-        if (lx > 0 && fg[i - 1]) l_unite(loc, i, i - 1);
-        if (ly > 0) {
-            if (lx > 0 && fg[i - kCrTw - 1]) l_unite(loc, i, i - kCrTw - 1);
-            if (fg[i - kCrTw]) l_unite(loc, i, i - kCrTw);
-            if (lx + 1 < kCrTw && fg[i - kCrTw + 1]) l_unite(loc, i, i - kCrTw + 1);
-        }
+    for (int ly = 1 + wave; ly < kCrTh; ly += kWaves) {
+        const uint64_t m = fgrow[ly], u = fgrow[ly - 1];
+        if (!((m >> lane) & 1)) continue;
+        const int i = ly * kCrTw + lane;
+        const bool start = lane == 0 || !((m >> (lane - 1)) & 1);
+        const bool end = lane == 63 || !((m >> (lane + 1)) & 1);
+        const bool n = (u >> lane) & 1;
+        const bool nw = lane > 0 && ((u >> (lane - 1)) & 1);
+        const bool ne = lane < 63 && ((u >> (lane + 1)) & 1);
+        if (n && (start || !nw)) l_unite(loc, i, i - kCrTw);
+        if (!n && start && nw) l_unite(loc, i, i - kCrTw - 1);
+        if (!n && end && ne) l_unite(loc, i, i - kCrTw + 1);
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        const int i = (int)threadIdx.x + kCrLocalThreads * k;
-        const int gx = x0 + (i & (kCrTw - 1)), gy = y0 + i / kCrTw;
+    for (int ly = wave; ly < kCrTh; ly += kWaves) {
+        const int gy = y0 + ly;
         if (gx >= a.W || gy >= a.H) continue;
-        const int root = l_find(loc, i);
+        const int root = l_find(loc, ly * kCrTw + lane);
         a.label[gy * a.W + gx] = (y0 + root / kCrTw) * a.W + x0 + (root & (kCrTw - 1));
     }
 }
 
-// Border pass: the pairs of 8-neighbours in different tiles.  Every such
-// pair has one pixel whose earlier neighbour (W, NW, N or NE) lies across
-// its tile's top row, left column or right column.
-__global__ void __launch_bounds__(256) cr_union_kernel(CrArgs a)
+// Border pass: the pairs of 8-neighbours in different tiles.  One wave per
+// tile edge: the top edge (64 pixels of the tile's first row) against the
+// row above, the left edge (64 pixels of its first column) against the
+// column to the left, each line extended by one pixel past both ends (the
+// corner diagonals).  Pixels along an edge are joined where they touch
+// inside their own tile, so the run rule of the tile pass applies with the
+// edge as the line: one union per pair of touching runs.  A right edge is
+// the next tile's left edge (its NE pairs are that edge's SW pairs).
+__global__ void __launch_bounds__(256) cr_union_kernel(CrArgs a, int tiles_x, int ntiles)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= a.W || y >= a.H) return;
-    const bool left = (x & (kCrTw - 1)) == 0, right = (x & (kCrTw - 1)) == kCrTw - 1;
-    const bool top = (y & (kCrTh - 1)) == 0;
-    if (!(left || right || top)) return;
-    if (!cr_fg(a, a.in[(int64_t)y * a.pitch + x])) return;
-    const int idx = y * a.W + x;
-    if (left && x > 0 && cr_fg(a, a.in[(int64_t)y * a.pitch + x - 1])) cr_unite(a.label, idx, idx - 1);
-    if (y > 0) {
-        const uint8_t *up = a.in + (int64_t)(y - 1) * a.pitch;
-        if ((left || top) && x > 0 && cr_fg(a, up[x - 1])) cr_unite(a.label, idx, idx - a.W - 1);
-        if (top && cr_fg(a, up[x])) cr_unite(a.label, idx, idx - a.W);
-        if ((top || right) && x + 1 < a.W && cr_fg(a, up[x + 1]))
-            cr_unite(a.label, idx, idx - a.W + 1);
-    }
+    const int lane = (int)threadIdx.x & 63;
+    const int e = (int)blockIdx.x * 4 + ((int)threadIdx.x >> 6);   // 2 * tile + (0 top, 1 left)
+    const int tile = e >> 1;
+    if (tile >= ntiles) return;                                  // whole waves
+    const int x0 = tile % tiles_x * kCrTw, y0 = tile / tiles_x * kCrTh;
+    const bool top = (e & 1) == 0;
+    if (top ? y0 == 0 : x0 == 0) return;
+    // this pixel (x, y) and its neighbour across the edge (xo, yo); the
+    // line's step from one lane to the next is (dx, dy)
+    const int x = top ? x0 + lane : x0, y = top ? y0 : y0 + lane;
+    const int dx = top ? 1 : 0, dy = top ? 0 : 1;
+    const int xo = top ? x : x - 1, yo = top ? y - 1 : y;
+    auto fg_at = [&](int px, int py) {
+        return px >= 0 && px < a.W && py >= 0 && py < a.H &&
+               cr_fg(a, a.in[(int64_t)py * a.pitch + px]);
+    };
+    const bool f = fg_at(x, y), o = fg_at(xo, yo);
+    const uint64_t fm = __ballot(f), om = __ballot(o);
+    // the other line's pixels before and after this one (past the ends: read)
+    const bool ob = lane > 0 ? ((om >> (lane - 1)) & 1) : fg_at(xo - dx, yo - dy);
+    const bool oa = lane < 63 ? ((om >> (lane + 1)) & 1) : fg_at(xo + dx, yo + dy);
+    if (!f) return;
+    const bool start = lane == 0 || !((fm >> (lane - 1)) & 1);
+    const bool end = lane == 63 || !((fm >> (lane + 1)) & 1);
+    const int idx = y * a.W + x, io = yo * a.W + xo, step = dy * a.W + dx;
+    if (o && (start || !ob)) cr_unite(a.label, idx, io);
+    if (!o && start && ob) cr_unite(a.label, idx, io - step);
+    if (!o && end && oa) cr_unite(a.label, idx, io + step);
 }
 
 __global__ void cr_compress_kernel(CrArgs a)
@@ -558,7 +587,12 @@ int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t
     hipLaunchKernelGGL(cr_init_kernel, dim3(flat), dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_local_kernel, dim3((W + kCrTw - 1) / kCrTw, (H + kCrTh - 1) / kCrTh),
                        dim3(kCrLocalThreads), 0, s, a);
-    hipLaunchKernelGGL(cr_union_kernel, g2, dim3(256), 0, s, a);
+    {
+        const int tiles_x = (W + kCrTw - 1) / kCrTw, tiles_y = (H + kCrTh - 1) / kCrTh;
+        const int edges = 2 * tiles_x * tiles_y;
+        hipLaunchKernelGGL(cr_union_kernel, dim3((edges + 3) / 4), dim3(256), 0, s, a, tiles_x,
+                           tiles_x * tiles_y);
+    }
     hipLaunchKernelGGL(cr_compress_kernel, dim3(flat), dim3(256), 0, s, a);
     if (a.covered > 0)
         hipLaunchKernelGGL(cr_area_kernel, dim3((a.covered + 3) / 4), dim3(256), 0, s, a);
