@@ -15,11 +15,11 @@
 // components of pixels whose gray values lie in [minGray, maxGray], each
 // labelled by its minimum raster index.  The reference gets there with
 // shared-memory labelling per block and rounds of border merging; here each
-// 64 x 16 tile is labelled in LDS, then the pairs across tile borders are
-// joined by a lock-free union-find pass in global memory (atomicMin on
-// roots, parents always smaller than children, so every loop ends), then
-// path compression -- the result does not depend on the order the unions
-// run in.  Areas, the area filter, the raster-order
+// 64 x 64 tile is labelled in LDS, then the runs that touch across tile
+// edges are joined by a lock-free union-find pass in global memory
+// (atomicMin on roots, parents always smaller than children, so every loop
+// ends), then path compression -- the result does not depend on the order
+// the unions run in.  Areas, the area filter, the raster-order
 // renumbering and the u8 output follow the reference, including its area
 // launch covering only rows below 256 * ceil(floor((H + 4) / 32) / 8)
 // (ConnectRegion.cu:45, :957-967) and never dropping a root whose counted
