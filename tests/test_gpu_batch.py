@@ -104,3 +104,20 @@ def test_full_c2_bit_exact(device):
     assert bc.batch.pixels == sum(it.pixels for it in bc.items)
     bc.batch.run()
     bc.check()
+
+
+@pytest.mark.parametrize("config", [-1, 0, 1])
+def test_unrolled_batch_every_ring_radius_both_modes(device, config):
+    # every radius with an unrolled kernel, REF and STD jobs in one launch
+    # (halo 16: the r = 15 instantiations of the batch kernel)
+    items, modes = [], []
+    for k, r in enumerate((3, 5, 7, 9, 15)):
+        for mode in (oc.MODE_REF, oc.MODE_STD):
+            W, H = 150 + 37 * k, 120 + 23 * k
+            items.append(wl.Item(W, H, 12, 6, 50, 10.0, 1000 + 10 * k + mode,
+                                 (3, 2, W - 1, H - 4), r, r / 1.5, 18.0 + 4 * k))
+            modes.append(mode)
+    bc = BatchCase(items, modes, config=config)
+    assert bc.batch.unrolled
+    bc.batch.run()
+    bc.check()
