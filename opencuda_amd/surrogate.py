@@ -137,7 +137,11 @@ class Dataset:
 
     def __init__(self, n_images: int = 2000, size: int = 512, radius: int = 15,
                  sigma_space: float = 10.0, sr_range=(15.0, 45.0), mode: int = MODE_STD,
-                 seed: int = 2024, log=None):
+                 seed: int = 2024, log=None, c3_frac: float = 0.0):
+        """c3_frac: fraction of the images drawn with the 8192^2 benchmark
+        image's generator settings scaled to `size` (its shape density,
+        shape sizes 16..1024, noise 12), the others with random settings
+        (10-60 shapes, sizes from 8-24 up to 16-320, noise 10 or 12)."""
         import torch
         rng = np.random.default_rng(seed)
         self.size = size
@@ -154,6 +158,9 @@ class Dataset:
             smin = int(rng.integers(8, 24))
             smax = int(rng.integers(smin + 8, 320))
             noise = float(rng.choice([10.0, 12.0]))
+            if rng.random() < c3_frac:
+                nshapes = max(1, round(1600 * size * size / (8192 * 8192)))
+                smin, smax, noise = 16, 1024, 12.0
             img = synth_image(size, size, nshapes, smin, smax, noise, int(rng.integers(1 << 62)))
             self.x[i].copy_(torch.from_numpy(img))
             sr = float(srs[i])
@@ -260,13 +267,15 @@ def _main() -> int:
     ap.add_argument("--log-every", type=int, default=100)
     ap.add_argument("--edge-frac", type=float, default=0.0,
                     help="fraction of crop offsets snapped to an image edge (trained unmasked)")
+    ap.add_argument("--c3-frac", type=float, default=0.0,
+                    help="fraction of training images with the 8192^2 benchmark image's settings")
     args = ap.parse_args()
 
     def log(m):
         print(f"[surrogate] {m}", file=sys.stderr, flush=True)
 
     mode = MODE_STD if args.mode == "std" else MODE_REF
-    data = Dataset(args.images, args.size, mode=mode, log=log)
+    data = Dataset(args.images, args.size, mode=mode, log=log, c3_frac=args.c3_frac)
     net = Surrogate(args.seed)
     init_steps = 0
     if args.init:
@@ -277,7 +286,7 @@ def _main() -> int:
     net.save(args.out, history=np.array(hist, np.float64), mode=mode, radius=15,
              sigma_space=10.0, images=args.images, steps=init_steps + args.steps, batch=args.batch,
              crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine),
-             init_steps=init_steps, edge_frac=args.edge_frac)
+             init_steps=init_steps, edge_frac=args.edge_frac, c3_frac=args.c3_frac)
     log(f"saved {args.out}")
     if args.eval:
         from . import workloads
