@@ -8,14 +8,34 @@
 //   (its ROI = whole image), else let doFilter create it.
 //   inplace = 1: doFilter(in) and write `in`.
 // Prints "status <code>" and exits 0 when the call returned NO_ERROR.
+//
+// usage: api_driver alloc-fail
+//   asks for a 1 TiB device image (more than the device holds): prints
+//   "device_huge <code>", then "device_after <code>" for a small image
+//   allocated next.  No kernel runs.
 #include <cstdio>
 #include <cstdlib>
 
 #include "BilateralFilter.h"
 #include "Image.h"
 
+#include <cstring>
+
+static int alloc_fail()
+{
+    Image *img = NULL;
+    ImageBasicOp::newImage(&img);
+    std::printf("device_huge %d\n",
+                ImageBasicOp::makeAtCurrentDevice(img, (size_t)1 << 20, (size_t)1 << 20));
+    std::printf("device_empty %d\n", img->imgData == NULL ? 1 : 0);
+    std::printf("device_after %d\n", ImageBasicOp::makeAtCurrentDevice(img, 64, 64));
+    ImageBasicOp::deleteImage(img);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc == 2 && std::strcmp(argv[1], "alloc-fail") == 0) return alloc_fail();
     if (argc != 15) {
         std::fprintf(stderr, "bad arguments\n");
         return 2;

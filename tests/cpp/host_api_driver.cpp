@@ -133,6 +133,15 @@ int main(int argc, char **argv)
     ImageBasicOp::deleteImage(im);
     ImageBasicOp::deleteImage(back);
 
+    // --- allocation failure: a 2^60-byte host image cannot be allocated;
+    // the image stays empty and the next allocation works
+    Image *huge = NULL;
+    ImageBasicOp::newImage(&huge);
+    out("make_host_huge", ImageBasicOp::makeAtHost(huge, (size_t)1 << 40, (size_t)1 << 20));
+    out("huge_still_empty", huge->imgData == NULL);
+    out("make_host_after", ImageBasicOp::makeAtHost(huge, 16, 16));
+    ImageBasicOp::deleteImage(huge);
+
     // --- BilateralFilter setters (okano_3_0/BilateralFilter.h:121-228)
     BilateralFilter bf;
     out("default_radius", bf.getRadius());

@@ -94,3 +94,15 @@ def test_zero_repeat_is_invalid(tmp_path, driver):
     img = oc.synth_image(40, 30, 3, 2, 9, 10.0, 6)
     status, out, p = _run(tmp_path, img, 3.0, 30.0, 3, 0, (0, 0, 40, 30))
     assert status == oc.INVALID_DATA
+
+
+def test_device_allocation_failure_is_reported(driver):
+    """A device image larger than the device (1 TiB) fails with CUDA_ERROR,
+    as the reference's cudaMallocPitch path does (Image.cu:94-137); the image
+    stays empty and the next allocation works.  No kernel runs."""
+    p = subprocess.run([DRIVER, "alloc-fail"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0, p.stderr
+    res = dict(line.split() for line in p.stdout.splitlines())
+    assert int(res["device_huge"]) == oc.CUDA_ERROR
+    assert int(res["device_empty"]) == 1
+    assert int(res["device_after"]) == 0

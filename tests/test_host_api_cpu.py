@@ -34,6 +34,7 @@ EXPECT = {
     "tplt_same_weights": 1, "tplt_missing": -4,
     "make_host": 0, "roi_default": 1, "bmp_write": 0, "bmp_read": 0, "bmp_same": 1,
     "bmp_missing": -4, "make_twice": -8,
+    "make_host_huge": -6, "huge_still_empty": 1, "make_host_after": 0,
     "roi_sub": 0, "roi_sub_w": 28, "roi_sub_h": 6, "roi_sub_off": 3 * 37 + 2,
     "default_radius": 0, "set_radius0": -1, "set_radius21": -1, "set_radius20": 0,
     "set_repeat0": -1, "set_sigma_neg": -1, "sigma_kept": 1, "set_sigma": 0, "set_range": 0,
