@@ -384,12 +384,32 @@ __global__ void __launch_bounds__(256) wgrad_kernel(WgradArgs a)
     if (tid < COUT) part[9 * CIN * COUT + tid] = dbias;
 }
 
+// The per-block partial gradients are summed in two passes with a fixed
+// order (training stays bit-reproducible): pass 1 (grid.y = run) sums each
+// run of kReduceRun partials into the run's first partial, pass 2 sums the
+// runs' totals.  One thread per (gradient entry, run) keeps 32x more loads in
+// flight than one serial sum over all partials.
+constexpr int kReduceRun = 32;
+
+__global__ void reduce_partial_runs(float *partial, int nparts, int psz)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= psz) return;
+    const int b0 = blockIdx.y * kReduceRun;
+    const int b1 = min(b0 + kReduceRun, nparts);
+    float s = 0.0f;
+#pragma unroll 8
+    for (int b = b0; b < b1; b++) s += partial[(size_t)b * psz + k];
+    partial[(size_t)b0 * psz + k] = s;
+}
+
 __global__ void reduce_partials(const float *partial, int nparts, int psz, float *grad)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= psz) return;
     float s = 0.0f;
-    for (int b = 0; b < nparts; b++) s += partial[(size_t)b * psz + k];
+#pragma unroll 8
+    for (int b = 0; b < nparts; b += kReduceRun) s += partial[(size_t)b * psz + k];
     grad[k] = s;
 }
 
@@ -1058,6 +1078,10 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
     hipLaunchKernelGGL(wgrad_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);
     BF_HIP(hipGetLastError());
     const int psz = pcount(layer);
+    const int runs = (blocks + kReduceRun - 1) / kReduceRun;
+    hipLaunchKernelGGL(reduce_partial_runs, dim3((psz + 255) / 256, runs), dim3(256), 0, s,
+                       net->partial, blocks, psz);
+    BF_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce_partials, dim3((psz + 255) / 256), dim3(256), 0, s, net->partial,
                        blocks, psz, net->grad + poff(layer));
     BF_HIP(hipGetLastError());
