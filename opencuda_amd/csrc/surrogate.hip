@@ -384,6 +384,156 @@ __global__ void __launch_bounds__(256) wgrad_kernel(WgradArgs a)
     if (tid < COUT) part[9 * CIN * COUT + tid] = dbias;
 }
 
+// Transposes 8 pixels x 8 channels of BF16 (v[j] = 8 channels of pixel j)
+// into out[q] = channel q of the 8 pixels, 16 bytes each.
+__device__ __forceinline__ void transpose8x8(const uint4 (&v)[8], uint4 (&out)[8])
+{
+    const uint32_t *w[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j] = reinterpret_cast<const uint32_t *>(&v[j]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        uint32_t lo[4], hi[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t a = w[2 * i][k], b = w[2 * i + 1][k];
+            lo[i] = (a & 0xFFFFu) | (b << 16);          // channel 2k of pixels 2i, 2i+1
+            hi[i] = (a >> 16) | (b & 0xFFFF0000u);      // channel 2k+1
+        }
+        out[2 * k] = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        out[2 * k + 1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    }
+}
+
+// Weight gradients of the 32-input-channel layers (hidden, last), same
+// result layout as wgrad_kernel.  The column shift of a tap is moved onto the
+// gradient: sum_x in[y+dy][x+dx] d[y][x] = sum_x' in[y+dy][x'] d[y][x'-dx],
+// with both factors zero outside the crop, so a segment of 64 input columns
+// x' needs its three input rows without a column halo and three copies of
+// the one gradient row shifted by dx = -1, 0, 1.  Both are transposed in
+// registers (8 pixels x 8 channels per thread) and written to LDS as 16-byte
+// rows, K = pixels contiguous for the MFMA operands.
+constexpr int WG_PITCH = TX + 8;   // LDS row of 64 pixels + 8 pad (bank spread)
+
+template <int KIND>
+__global__ void __launch_bounds__(256) wgrad32_kernel(WgradArgs a)
+{
+    static_assert(KIND == K_HIDDEN || KIND == K_LAST, "32-channel inputs only");
+    constexpr int CIN = C;
+    constexpr int COUT = (KIND == K_LAST) ? 1 : C;
+    constexpr int M = 9 * CIN;
+    constexpr int MT = M / 16;                    // one tap per M-tile
+    constexpr int NT = (COUT + 15) / 16;
+    constexpr int PAIRS = MT * NT;
+    constexpr int PPW = (PAIRS + 3) / 4;
+    constexpr int PSZ = 9 * CIN * COUT + COUT;
+    constexpr int DROWS = NT * 16;
+    __shared__ __align__(16) __bf16 s_in[3][CIN][WG_PITCH];
+    __shared__ __align__(16) __bf16 s_d[3][DROWS][WG_PITCH];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i16 = lane & 15, kq = lane >> 4;
+    const int segs_x = (a.W + TX - 1) / TX;
+    const int64_t nseg = (int64_t)a.N * a.H * segs_x;
+    const int64_t npix = (int64_t)a.H * a.W;
+
+    if (KIND == K_LAST) {   // gradient rows 1..15 of the single output channel stay zero
+        for (int e = tid; e < 3 * 15 * TX; e += 256) {
+            const int dxi = e / (15 * TX), rem = e - dxi * 15 * TX;
+            s_d[dxi][1 + rem / TX][rem % TX] = to_bf(0.0f);
+        }
+    }
+
+    f32x4 acc[PPW];
+#pragma unroll
+    for (int q = 0; q < PPW; q++) acc[q] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
+    float dbias = 0.0f;
+
+    for (int64_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const int n = (int)(seg / ((int64_t)a.H * segs_x));
+        const int rem = (int)(seg - (int64_t)n * a.H * segs_x);
+        const int y = rem / segs_x, x0 = (rem - y * segs_x) * TX;
+        __syncthreads();
+        if (tid < 96) {
+            // input rows y-1..y+1, columns x0..x0+63: task (row, channel chunk, 8-pixel group)
+            const int r = tid >> 5, chunk = (tid >> 3) & 3, xg = tid & 7;
+            const int gy = y - 1 + r;
+            uint4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int gx = x0 + 8 * xg + j;
+                v[j] = make_uint4(0, 0, 0, 0);
+                if (gy >= 0 && gy < a.H && gx < a.W)
+                    v[j] = *reinterpret_cast<const uint4 *>(a.act + ((size_t)n * npix + (size_t)gy * a.W + gx) * C + chunk * 8);
+            }
+            uint4 t[8];
+            transpose8x8(v, t);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                *reinterpret_cast<uint4 *>(&s_in[r][chunk * 8 + q][8 * xg]) = t[q];
+        } else if (KIND == K_HIDDEN && tid < 192) {
+            // gradient row y shifted by dx: copy dxi holds d[y][x' - dx] at column x'
+            const int u = tid - 96;
+            const int dxi = u >> 5, chunk = (u >> 3) & 3, xg = u & 7;
+            uint4 v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int gx = x0 + 8 * xg + j - (dxi - 1);
+                v[j] = make_uint4(0, 0, 0, 0);
+                if (gx >= 0 && gx < a.W)
+                    v[j] = *reinterpret_cast<const uint4 *>(a.dpre + ((size_t)n * npix + (size_t)y * a.W + gx) * C + chunk * 8);
+            }
+            uint4 t[8];
+            transpose8x8(v, t);
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                *reinterpret_cast<uint4 *>(&s_d[dxi][chunk * 8 + q][8 * xg]) = t[q];
+        }
+        if (KIND == K_LAST) {
+            for (int e = tid; e < 3 * TX; e += 256) {
+                const int dxi = e / TX, xx = e - dxi * TX;
+                const int gx = x0 + xx - (dxi - 1);
+                __bf16 v = to_bf(0.0f);
+                if (gx >= 0 && gx < a.W) v = a.dpre[(size_t)n * npix + (size_t)y * a.W + gx];
+                s_d[dxi][0][xx] = v;
+            }
+        }
+        __syncthreads();
+        if (tid < COUT) {   // the unshifted copy is gradient row y over columns x0..x0+63
+            float sacc = 0.0f;
+            for (int xx = 0; xx < TX; xx++) sacc += from_bf(s_d[1][tid][xx]);
+            dbias += sacc;
+        }
+#pragma unroll
+        for (int q = 0; q < PPW; q++) {
+            const int pair = wave * PPW + q;
+            if (pair >= PAIRS) break;
+            const int mt = pair / NT, nt = pair - mt * NT;
+            const int t = (16 * mt) / CIN, ci = (16 * mt) % CIN + i16;
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+#pragma unroll
+            for (int ks = 0; ks < TX / 32; ks++) {
+                const bf16x8 av = *reinterpret_cast<const bf16x8 *>(&s_in[dy + 1][ci][32 * ks + 8 * kq]);
+                const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&s_d[dx + 1][16 * nt + i16][32 * ks + 8 * kq]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q], 0, 0, 0);
+            }
+        }
+    }
+    float *part = a.partial + (size_t)blockIdx.x * PSZ;
+#pragma unroll
+    for (int q = 0; q < PPW; q++) {
+        const int pair = wave * PPW + q;
+        if (pair >= PAIRS) break;
+        const int mt = pair / NT, nt = pair - mt * NT;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = 16 * mt + 4 * kq + r, co = 16 * nt + i16;
+            if (co < COUT) part[m * COUT + co] = acc[q][r];
+        }
+    }
+    if (tid < COUT) part[9 * CIN * COUT + tid] = dbias;
+}
+
 // The per-block partial gradients are summed in two passes with a fixed
 // order (training stays bit-reproducible): pass 1 (grid.y = run) sums each
 // run of kReduceRun partials into the run's first partial, pass 2 sums the
@@ -1075,7 +1225,10 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
     w.partial = net->partial;
     const int64_t nseg = (int64_t)w.N * w.H * ((w.W + TX - 1) / TX);
     const int blocks = (int)std::min<int64_t>(net->partial_blocks, nseg);
-    hipLaunchKernelGGL(wgrad_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);
+    if constexpr (KIND == K_FIRST)
+        hipLaunchKernelGGL(wgrad_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);
+    else
+        hipLaunchKernelGGL(wgrad32_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);
     BF_HIP(hipGetLastError());
     const int psz = pcount(layer);
     const int runs = (blocks + kReduceRun - 1) / kReduceRun;
