@@ -83,6 +83,14 @@ struct KTraits {
     static constexpr bool STAGE32 = (KIND == K_HIDDEN || KIND == K_LAST || KIND == K_DG_HIDDEN);
 };
 
+// Two BF16 gradients with the ReLU mask of the matching activations: a half
+// is kept where that activation is > 0, else it becomes +0.
+__device__ __forceinline__ uint32_t relu_keep(uint32_t v, uint32_t act)
+{
+    const float lo = __uint_as_float(act << 16), hi = __uint_as_float(act & 0xFFFF0000u);
+    return v & ((lo > 0.0f ? 0x0000FFFFu : 0u) | (hi > 0.0f ? 0xFFFF0000u : 0u));
+}
+
 // Implicit-GEMM 3x3 convolution: out[p][co] = sum_{t,ci} in[p+off(t)][ci] * W[t][ci][co].
 // Workgroup = 4 waves = 64 x 4 output pixels; wave w owns row y0 + w as 4
 // M-tiles of 16 pixels.
@@ -90,7 +98,10 @@ template <int KIND>
 __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
 {
     using T = KTraits<KIND>;
-    __shared__ __align__(16) __bf16 s_in[T::STAGE32 ? (TY + 2) * (TX + 2) * PITCH : 8];
+    // Staged input tile; after the MFMAs it also holds the output tile (the
+    // epilogue's [wave][64 pixels][PITCH] rows) for 16-byte global stores.
+    __shared__ __align__(16) __bf16 s_in[T::STAGE32 ? (TY + 2) * (TX + 2) * PITCH
+                                                   : (KIND == K_LAST ? 8 : TY * TX * PITCH)];
     __shared__ float s_first[(KIND == K_FIRST || KIND == K_DG_LAST) ? (TY + 2) * (TX + 2) : 1];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -201,25 +212,37 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
 
     // ---- epilogue
     const int y = y0 + wave;
-    if (y < H) {
+    if constexpr (KIND == K_LAST) {
+        if (y < H) {
 #pragma unroll
-    for (int m = 0; m < 4; m++) {
+            for (int m = 0; m < 4; m++) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int x = x0 + 16 * m + 4 * kq + r;
-            if (x >= W) continue;
-            const size_t p = (size_t)n * npix + (size_t)y * W + x;
-            if (KIND == K_LAST) {
-                if (i16 != 0) continue;
-                const uint8_t u = a.img[(size_t)n * a.img_stride + (int64_t)y * a.img_pitch + x];
-                const float v = (float)u / 255.0f + (acc[m][0][r] + a.bias[0]);
-                if (a.pred) a.pred[p] = v;
-                if (a.out_u8) {
-                    const float q = rintf(v * 255.0f);
-                    a.out_u8[(int64_t)y * a.u8_pitch + x] =
-                        (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
+                for (int r = 0; r < 4; r++) {
+                    const int x = x0 + 16 * m + 4 * kq + r;
+                    if (x >= W || i16 != 0) continue;
+                    const size_t p = (size_t)n * npix + (size_t)y * W + x;
+                    const uint8_t u = a.img[(size_t)n * a.img_stride + (int64_t)y * a.img_pitch + x];
+                    const float v = (float)u / 255.0f + (acc[m][0][r] + a.bias[0]);
+                    if (a.pred) a.pred[p] = v;
+                    if (a.out_u8) {
+                        const float q = rintf(v * 255.0f);
+                        a.out_u8[(int64_t)y * a.u8_pitch + x] =
+                            (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
+                    }
                 }
-            } else {
+            }
+        }
+    } else {
+        // NHWC-32 output: each wave writes its 64 x 32 tile to LDS (the
+        // staged input is dead once every wave has done its MFMAs), then
+        // stores it as 16 contiguous bytes per lane, the data-gradient kinds
+        // reading the ReLU mask the same way.
+        __syncthreads();
+        __bf16 *s_out = s_in + wave * TX * PITCH;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
 #pragma unroll
                 for (int nt = 0; nt < T::NT; nt++) {
                     const int co = 16 * nt + i16;
@@ -227,14 +250,28 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
                     if (KIND == K_FIRST || KIND == K_HIDDEN) {
                         v += a.bias[co];
                         v = v > 0.0f ? v : 0.0f;
-                    } else {
-                        if (!(from_bf(a.mask[p * C + co]) > 0.0f)) v = 0.0f;
                     }
-                    a.out[p * C + co] = to_bf(v);
+                    s_out[(16 * m + 4 * kq + r) * PITCH + co] = to_bf(v);
                 }
+        __syncthreads();
+        if (y < H) {
+            const int chunk = lane & 3;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int px = 16 * j + (lane >> 2), x = x0 + px;
+                if (x >= W) continue;
+                const size_t p = (size_t)n * npix + (size_t)y * W + x;
+                uint4 v = *reinterpret_cast<const uint4 *>(&s_out[px * PITCH + chunk * 8]);
+                if (KIND == K_DG_HIDDEN || KIND == K_DG_LAST) {
+                    const uint4 mk = *reinterpret_cast<const uint4 *>(a.mask + p * C + chunk * 8);
+                    v.x = relu_keep(v.x, mk.x);
+                    v.y = relu_keep(v.y, mk.y);
+                    v.z = relu_keep(v.z, mk.z);
+                    v.w = relu_keep(v.w, mk.w);
+                }
+                *reinterpret_cast<uint4 *>(a.out + p * C + chunk * 8) = v;
             }
         }
-    }
     }
     }   // tile loop
 }
