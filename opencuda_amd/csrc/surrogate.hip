@@ -453,7 +453,7 @@ __device__ __forceinline__ void transpose8x8(const uint4 (&v)[8], uint4 (&out)[8
 constexpr int WG_PITCH = TX + 8;   // LDS row of 64 pixels + 8 pad (bank spread)
 
 template <int KIND>
-__global__ void __launch_bounds__(256) wgrad32_kernel(WgradArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) wgrad32_kernel(WgradArgs a)
 {
     static_assert(KIND == K_HIDDEN || KIND == K_LAST, "32-channel inputs only");
     constexpr int CIN = C;
@@ -467,12 +467,56 @@ __global__ void __launch_bounds__(256) wgrad32_kernel(WgradArgs a)
     constexpr int DROWS = NT * 16;
     __shared__ __align__(16) __bf16 s_in[3][CIN][WG_PITCH];
     __shared__ __align__(16) __bf16 s_d[3][DROWS][WG_PITCH];
+    __shared__ float s_bias[64];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i16 = lane & 15, kq = lane >> 4;
     const int segs_x = (a.W + TX - 1) / TX;
     const int64_t nseg = (int64_t)a.N * a.H * segs_x;
     const int64_t npix = (int64_t)a.H * a.W;
+
+    // Staging tasks.  Threads 0..95: input row r = y-1+(tid>>5), channel
+    // chunk (tid>>3)&3, pixels 8*(tid&7)..+7.  Hidden, threads 96..191:
+    // gradient copy dxi = (u>>5), same chunk/pixel split.  Last layer,
+    // threads 0..191: one pixel of one gradient copy (dxi = tid / 64).
+    const bool in_task = tid < 96;
+    const bool d_task = (KIND == K_HIDDEN) ? (tid >= 96 && tid < 192) : (tid < 192);
+    const int u = (KIND == K_HIDDEN) ? tid - 96 : tid;
+    const int t_row = tid >> 5, t_chunk = (tid >> 3) & 3, t_xg = tid & 7;
+    const int d_dxi = (KIND == K_HIDDEN) ? (u >> 5) : (u >> 6);
+    const int d_chunk = (u >> 3) & 3, d_xg = u & 7, d_xx = u & 63;
+
+    uint4 v[8];   // the thread's 8 pixels x 8 channels (input or gradient task)
+    __bf16 vd1 = to_bf(0.0f);
+    auto load_seg = [&](int64_t seg) {
+        const int n = (int)(seg / ((int64_t)a.H * segs_x));
+        const int rem = (int)(seg - (int64_t)n * a.H * segs_x);
+        const int y = rem / segs_x, x0 = (rem - y * segs_x) * TX;
+        if (in_task) {
+            const int gy = y - 1 + t_row;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int gx = x0 + 8 * t_xg + j;
+                v[j] = make_uint4(0, 0, 0, 0);
+                if (gy >= 0 && gy < a.H && gx < a.W)
+                    v[j] = *reinterpret_cast<const uint4 *>(a.act + ((size_t)n * npix + (size_t)gy * a.W + gx) * C + t_chunk * 8);
+            }
+        }
+        if (KIND == K_HIDDEN && d_task) {
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int gx = x0 + 8 * d_xg + j - (d_dxi - 1);
+                v[j] = make_uint4(0, 0, 0, 0);
+                if (gx >= 0 && gx < a.W)
+                    v[j] = *reinterpret_cast<const uint4 *>(a.dpre + ((size_t)n * npix + (size_t)y * a.W + gx) * C + d_chunk * 8);
+            }
+        }
+        if (KIND == K_LAST && d_task) {
+            const int gx = x0 + d_xx - (d_dxi - 1);
+            vd1 = to_bf(0.0f);
+            if (gx >= 0 && gx < a.W) vd1 = a.dpre[(size_t)n * npix + (size_t)y * a.W + gx];
+        }
+    };
 
     if (KIND == K_LAST) {   // gradient rows 1..15 of the single output channel stay zero
         for (int e = tid; e < 3 * 15 * TX; e += 256) {
@@ -484,63 +528,45 @@ __global__ void __launch_bounds__(256) wgrad32_kernel(WgradArgs a)
     f32x4 acc[PPW];
 #pragma unroll
     for (int q = 0; q < PPW; q++) acc[q] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
-    float dbias = 0.0f;
+    // bias gradient, accumulated by the unshifted gradient copy's tasks:
+    // hidden, 8 channels of 8 pixels per thread; last, one pixel per thread
+    float bsum[8] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
 
+    if ((int64_t)blockIdx.x < nseg) load_seg(blockIdx.x);
     for (int64_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const int n = (int)(seg / ((int64_t)a.H * segs_x));
-        const int rem = (int)(seg - (int64_t)n * a.H * segs_x);
-        const int y = rem / segs_x, x0 = (rem - y * segs_x) * TX;
-        __syncthreads();
-        if (tid < 96) {
-            // input rows y-1..y+1, columns x0..x0+63: task (row, channel chunk, 8-pixel group)
-            const int r = tid >> 5, chunk = (tid >> 3) & 3, xg = tid & 7;
-            const int gy = y - 1 + r;
-            uint4 v[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int gx = x0 + 8 * xg + j;
-                v[j] = make_uint4(0, 0, 0, 0);
-                if (gy >= 0 && gy < a.H && gx < a.W)
-                    v[j] = *reinterpret_cast<const uint4 *>(a.act + ((size_t)n * npix + (size_t)gy * a.W + gx) * C + chunk * 8);
-            }
+        __syncthreads();   // the previous segment's MFMAs are done with the LDS rows
+        if (in_task) {
             uint4 t[8];
             transpose8x8(v, t);
 #pragma unroll
             for (int q = 0; q < 8; q++)
-                *reinterpret_cast<uint4 *>(&s_in[r][chunk * 8 + q][8 * xg]) = t[q];
-        } else if (KIND == K_HIDDEN && tid < 192) {
-            // gradient row y shifted by dx: copy dxi holds d[y][x' - dx] at column x'
-            const int u = tid - 96;
-            const int dxi = u >> 5, chunk = (u >> 3) & 3, xg = u & 7;
-            uint4 v[8];
-#pragma unroll
-            for (int j = 0; j < 8; j++) {
-                const int gx = x0 + 8 * xg + j - (dxi - 1);
-                v[j] = make_uint4(0, 0, 0, 0);
-                if (gx >= 0 && gx < a.W)
-                    v[j] = *reinterpret_cast<const uint4 *>(a.dpre + ((size_t)n * npix + (size_t)y * a.W + gx) * C + chunk * 8);
-            }
+                *reinterpret_cast<uint4 *>(&s_in[t_row][t_chunk * 8 + q][8 * t_xg]) = t[q];
+        }
+        if (KIND == K_HIDDEN && d_task) {
             uint4 t[8];
             transpose8x8(v, t);
 #pragma unroll
             for (int q = 0; q < 8; q++)
-                *reinterpret_cast<uint4 *>(&s_d[dxi][chunk * 8 + q][8 * xg]) = t[q];
-        }
-        if (KIND == K_LAST) {
-            for (int e = tid; e < 3 * TX; e += 256) {
-                const int dxi = e / TX, xx = e - dxi * TX;
-                const int gx = x0 + xx - (dxi - 1);
-                __bf16 v = to_bf(0.0f);
-                if (gx >= 0 && gx < a.W) v = a.dpre[(size_t)n * npix + (size_t)y * a.W + gx];
-                s_d[dxi][0][xx] = v;
+                *reinterpret_cast<uint4 *>(&s_d[d_dxi][d_chunk * 8 + q][8 * d_xg]) = t[q];
+            if (d_dxi == 1) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    const uint32_t *w = reinterpret_cast<const uint32_t *>(&t[q]);
+                    float sq = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        sq += __uint_as_float(w[k] << 16) + __uint_as_float(w[k] & 0xFFFF0000u);
+                    bsum[q] += sq;
+                }
             }
         }
-        __syncthreads();
-        if (tid < COUT) {   // the unshifted copy is gradient row y over columns x0..x0+63
-            float sacc = 0.0f;
-            for (int xx = 0; xx < TX; xx++) sacc += from_bf(s_d[1][tid][xx]);
-            dbias += sacc;
+        if (KIND == K_LAST && d_task) {
+            s_d[d_dxi][0][d_xx] = vd1;
+            if (d_dxi == 1) bsum[0] += from_bf(vd1);
         }
+        __syncthreads();
+        // the next segment's loads are in flight during this segment's MFMAs
+        if (seg + gridDim.x < nseg) load_seg(seg + gridDim.x);
 #pragma unroll
         for (int q = 0; q < PPW; q++) {
             const int pair = wave * PPW + q;
@@ -568,7 +594,31 @@ __global__ void __launch_bounds__(256) wgrad32_kernel(WgradArgs a)
             if (co < COUT) part[m * COUT + co] = acc[q][r];
         }
     }
-    if (tid < COUT) part[9 * CIN * COUT + tid] = dbias;
+    // bias: sum the unshifted copy's per-thread sums in a fixed order
+    if (KIND == K_HIDDEN) {
+        if (d_task && d_dxi == 1) {
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (d_xg == 0) s_bias[d_chunk * 8 + q] = 0.0f;
+        }
+        __syncthreads();
+        for (int xg = 0; xg < 8; xg++) {
+            if (d_task && d_dxi == 1 && d_xg == xg) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) s_bias[d_chunk * 8 + q] += bsum[q];
+            }
+            __syncthreads();
+        }
+        if (tid < COUT) part[9 * CIN * COUT + tid] = s_bias[tid];
+    } else {
+        if (d_task && d_dxi == 1) s_bias[d_xx] = bsum[0];
+        __syncthreads();
+        if (tid == 0) {
+            float b = 0.0f;
+            for (int xx = 0; xx < 64; xx++) b += s_bias[xx];
+            part[9 * CIN * COUT] = b;
+        }
+    }
 }
 
 // The per-block partial gradients are summed in two passes with a fixed
@@ -1165,18 +1215,29 @@ int pack(Net *net, hipStream_t s)
     return kOk;
 }
 
+// Workgroups of 256 threads of `kern` resident on the whole device at once.
+int resident_blocks(const void *kern)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0) != hipSuccess)
+        return -1;
+    return cus * per_cu;
+}
+
 template <int KIND>
 int launch_conv(const ConvArgs &a, hipStream_t s)
 {
-    // Persistent: about 4 workgroups per CU walk the 64 x 4 tiles.
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        BF_HIP(hipGetDevice(&dev));
-        BF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    // Persistent: as many workgroups as are resident at once walk the 64 x 4
+    // tiles (a workgroup beyond that would start only when one has finished).
+    static int resident = 0;
+    if (!resident) {
+        resident = resident_blocks(reinterpret_cast<const void *>(&conv_kernel<KIND>));
+        if (resident <= 0) return kHipError;
     }
     const int64_t ntiles = (int64_t)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.N;
-    const int grid = (int)std::min<int64_t>(ntiles, (int64_t)cus * 4);
+    const int grid = (int)std::min<int64_t>(ntiles, resident);
     hipLaunchKernelGGL(conv_kernel<KIND>, dim3(grid), dim3(256), 0, s, a);
     BF_HIP(hipGetLastError());
     return kOk;
@@ -1261,7 +1322,15 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
     WgradArgs w = w0;
     w.partial = net->partial;
     const int64_t nseg = (int64_t)w.N * w.H * ((w.W + TX - 1) / TX);
-    const int blocks = (int)std::min<int64_t>(net->partial_blocks, nseg);
+    static int resident = 0;
+    if (!resident) {
+        if constexpr (KIND == K_FIRST)
+            resident = resident_blocks(reinterpret_cast<const void *>(&wgrad_kernel<KIND>));
+        else
+            resident = resident_blocks(reinterpret_cast<const void *>(&wgrad32_kernel<KIND>));
+        if (resident <= 0) return kHipError;
+    }
+    const int blocks = (int)std::min<int64_t>(std::min(net->partial_blocks, resident), nseg);
     if constexpr (KIND == K_FIRST)
         hipLaunchKernelGGL(wgrad_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);
     else
