@@ -29,7 +29,7 @@ from .surrogate import DEFAULT_WEIGHTS, Surrogate
 
 class Mixed:
     def __init__(self, items, net: Surrogate, mode: int = MODE_STD, sur_min_radius: int = 9,
-                 tol: int = 2, streams: int = 4, log=None):
+                 tol: int = 2, streams: int = 4, log=None, sur_min_pixels: int = 0):
         import torch
         self.items = items
         self.mode = mode
@@ -57,7 +57,8 @@ class Mixed:
             key = (it.sigma_space, it.sigma_range)
             if key not in self.luts:
                 self.luts[key] = Lut.from_sigmas(*key)
-            (self.sur_idx if it.radius >= sur_min_radius else self.ex_idx).append(i)
+            to_sur = it.radius >= sur_min_radius and it.pixels >= sur_min_pixels
+            (self.sur_idx if to_sur else self.ex_idx).append(i)
             if log and (i + 1) % 128 == 0:
                 log(f"{i + 1}/{len(items)} images uploaded ({time.time() - t0:.1f} s)")
         jobs = [Job(view(self.tin[i], items[i].roi), view(self.tout[i]),
@@ -128,11 +129,13 @@ class Mixed:
         return ref
 
 
-def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None):
+def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None,
+            sur_min_pixels=0):
     import torch
     net = Surrogate(seed=1)
     net.load(weights or DEFAULT_WEIGHTS)
-    mx = Mixed(wl.c4(count), net, mode=mode, streams=streams, log=log)
+    mx = Mixed(wl.c4(count), net, mode=mode, streams=streams, log=log,
+               sur_min_pixels=sur_min_pixels)
     if log:
         log(f"{len(mx.ex_idx)} exact images, {len(mx.sur_idx)} surrogate images, "
             f"{mx.pixels / 1e6:.1f} Mpx; capturing")
@@ -160,7 +163,7 @@ def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None
         "workload": f"c4: {count} images 128..4096 per side, r in {{3,7,15}}, "
                     f"{'STD' if mode == MODE_STD else 'REF'} mode",
         "images": count, "exact_images": len(mx.ex_idx), "surrogate_images": len(mx.sur_idx),
-        "streams": len(mx.nets),
+        "streams": len(mx.nets), "sur_min_pixels": sur_min_pixels,
         "pixels": mx.pixels, "ms_per_graph": round(ms, 3),
         "mpix_s": round(mx.pixels / (ms / 1e3) / 1e6, 1),
         "tiles_checked": tiles // steps, "tiles_fallback": fb // steps,
