@@ -9,7 +9,8 @@
 // The reference addresses every read as data + y * pitch + x from the ROI
 // origin and its walks reach 2 (pntRange - 1) pixels from the centre while
 // the guard band is pntRange + 2 wide; reads here use the same linear
-// address, clamped into the allocation the caller declares.
+// address, clamped into the allocation the caller declares.  A workgroup
+// stages its 64 x 16 tile plus the walks' reach in LDS by that rule.
 //
 // ConnectRegion (okano_3_0/ConnectRegion.cu:789-1043): 8-connected
 // components of pixels whose gray values lie in [minGray, maxGray], each
@@ -63,19 +64,15 @@ __device__ __forceinline__ uint8_t lc_sat_u8(float q)
     return (uint8_t)(int)q;
 }
 
-__global__ void __launch_bounds__(256) local_cluster_kernel(LcArgs a)
+// The filter of one pixel (x, y) whose value is cv; rd(dx, dy) returns the
+// byte at (x + dx, y + dy) by the linear-address rule of lc_read.
+template <typename Read>
+__device__ __forceinline__ uint8_t lc_pixel(const LcArgs &a, int x, int y, uint32_t cv, Read rd)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (x >= a.W || y >= a.H) return;
-    const uint32_t cv = lc_read(a, y, x);
     const int sx = a.pnt_range + 2, ex = a.W - a.pnt_range - 2;
     const int sy = a.pnt_range + 2, ey = a.H - a.pnt_range - 2;
-    uint8_t *dst = a.out + (int64_t)y * a.out_pitch + x;
-    if (x < sx || x > ex || y < sy || y > ey || (int)cv <= a.pro_black || (int)cv >= a.pro_white) {
-        *dst = (uint8_t)cv;
-        return;
-    }
+    if (x < sx || x > ex || y < sy || y > ey || (int)cv <= a.pro_black || (int)cv >= a.pro_white)
+        return (uint8_t)cv;
     // directions {dx, dy} of LocalCluster.cu:14-19
     const int ddx_t[8] = { 1, 1, 0, -1, -1, -1, 0, 1 };
     const int ddy_t[8] = { 0, -1, -1, -1, 0, 1, 1, 1 };
@@ -84,17 +81,16 @@ __global__ void __launch_bounds__(256) local_cluster_kernel(LcArgs a)
 #pragma unroll
     for (int z = 0; z < 8; z++) {
         const int ddx = ddx_t[z], ddy = ddy_t[z];
-        uint32_t cuv = lc_read(a, y + ddy, x + ddx);
-        uint32_t pre1 = lc_read(a, y - ddy, x - ddx);
+        uint32_t cuv = rd(ddx, ddy);
+        uint32_t pre1 = rd(-ddx, -ddy);
         uint32_t pre2 = cv;
-        uint32_t latt1 = lc_read(a, y + 2 * ddy, x + 2 * ddx);
+        uint32_t latt1 = rd(2 * ddx, 2 * ddy);
         float sum = fcv;
-        int i, px = x, py = y;
+        int i, k = 0;
         for (i = 1; i < a.pnt_range; i++) {
             if (fabsf(fcv - (float)cuv) > a.gap) break;
-            px += 2 * ddx;
-            py += 2 * ddy;
-            const uint32_t latt2 = lc_read(a, py, px);
+            k += 2;
+            const uint32_t latt2 = rd(k * ddx, k * ddy);
             const float side1 = (float)(pre1 + pre2);
             const float side2 = (float)(latt1 + latt2);
             if (fabsf(side1 - side2) > a.diffe) break;
@@ -119,7 +115,48 @@ __global__ void __launch_bounds__(256) local_cluster_kernel(LcArgs a)
     for (int j = 0; j < 8; j++) sumag += temp[j];
     sumag -= (float)((8 - a.pnt_count) * (int)cv);
     const float q = sumag / (float)a.pnt_count;
-    *dst = q > 255.0f ? 255 : lc_sat_u8(q);
+    return q > 255.0f ? 255 : lc_sat_u8(q);
+}
+
+// Every read straight from global memory (any pntRange).
+__global__ void __launch_bounds__(256) local_cluster_kernel(LcArgs a)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.W || y >= a.H) return;
+    a.out[(int64_t)y * a.out_pitch + x] =
+        lc_pixel(a, x, y, lc_read(a, y, x),
+                 [&](int dx, int dy) { return lc_read(a, y + dy, x + dx); });
+}
+
+// A 64 x 16 tile with a halo of `halo` = 2 (pntRange - 1) pixels (the walks'
+// reach) staged in LDS through lc_read, so every read the walks make sees
+// the byte the global-read kernel would.  Each thread filters 4 pixels of a
+// column (rows ty, ty + 4, ty + 8, ty + 12).
+constexpr int kLcTw = 64, kLcTh = 16;
+constexpr int64_t kLcMaxLds = 48 * 1024;   // larger halos read global memory
+
+__global__ void __launch_bounds__(256) local_cluster_lds_kernel(LcArgs a, int halo)
+{
+    extern __shared__ uint8_t s_img[];
+    const int sw = kLcTw + 2 * halo, sh = kLcTh + 2 * halo;
+    const int x0 = blockIdx.x * kLcTw, y0 = blockIdx.y * kLcTh;
+    for (int e = threadIdx.x; e < sw * sh; e += 256) {
+        const int r = e / sw, c = e - r * sw;
+        s_img[e] = (uint8_t)lc_read(a, y0 - halo + r, x0 - halo + c);
+    }
+    __syncthreads();
+    const int x = x0 + (threadIdx.x & 63);
+    if (x >= a.W) return;
+#pragma unroll 1
+    for (int k = 0; k < kLcTh / 4; k++) {
+        const int y = y0 + (threadIdx.x >> 6) + 4 * k;
+        if (y >= a.H) break;
+        const int base = (y - y0 + halo) * sw + (x - x0 + halo);
+        a.out[(int64_t)y * a.out_pitch + x] =
+            lc_pixel(a, x, y, s_img[base],
+                     [&](int dx, int dy) { return (uint32_t)s_img[base + dy * sw + dx]; });
+    }
 }
 
 // ----------------------------------------------------------- ConnectRegion
@@ -538,8 +575,15 @@ int bf_local_cluster(const bf_image_view *in, int64_t alloc_lo, int64_t alloc_hi
     a.diffe = (float)p->diffe_thred;
     a.pro_black = p->pro_black;
     a.pro_white = p->pro_white;
-    hipLaunchKernelGGL(local_cluster_kernel, dim3((a.W + 63) / 64, (a.H + 3) / 4), dim3(256), 0,
-                       (hipStream_t)stream, a);
+    // the walks reach 2 (pntRange - 1) pixels from the centre, and 2 at least
+    const int64_t halo = std::max<int64_t>(2, 2 * ((int64_t)a.pnt_range - 1));
+    const int64_t lds = (kLcTw + 2 * halo) * (kLcTh + 2 * halo);
+    if (lds <= kLcMaxLds)
+        hipLaunchKernelGGL(local_cluster_lds_kernel, dim3((a.W + kLcTw - 1) / kLcTw, (a.H + kLcTh - 1) / kLcTh),
+                           dim3(256), (size_t)lds, (hipStream_t)stream, a, (int)halo);
+    else
+        hipLaunchKernelGGL(local_cluster_kernel, dim3((a.W + 63) / 64, (a.H + 3) / 4), dim3(256), 0,
+                           (hipStream_t)stream, a);
     BF_HIP(hipGetLastError());
     return kOk;
 }

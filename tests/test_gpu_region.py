@@ -79,6 +79,10 @@ def test_connect_region_one_large_region(device):
     dict(pnt_range=8, pnt_count=5, gap_thred=10, diffe_thred=10),     # parameterised ctor
     dict(pnt_range=16, pnt_count=7, gap_thred=40, diffe_thred=40),    # default ctor
     dict(pnt_range=3, pnt_count=8, gap_thred=255, diffe_thred=255),
+    # halo 78: the LDS tile (220 x 172 bytes) near its size limit
+    dict(pnt_range=40, pnt_count=6, gap_thred=60, diffe_thred=60),
+    # halo 118: beyond the LDS limit, every read from global memory
+    dict(pnt_range=60, pnt_count=4, gap_thred=255, diffe_thred=255),
 ])
 @pytest.mark.parametrize("layout", ["full", "roi"])
 def test_local_cluster_matches_oracle(device, params, layout):
