@@ -141,9 +141,15 @@ __global__ void __launch_bounds__(256) local_cluster_lds_kernel(LcArgs a, int ha
     extern __shared__ uint8_t s_img[];
     const int sw = kLcTw + 2 * halo, sh = kLcTh + 2 * halo;
     const int x0 = blockIdx.x * kLcTw, y0 = blockIdx.y * kLcTh;
-    for (int e = threadIdx.x; e < sw * sh; e += 256) {
-        const int r = e / sw, c = e - r * sw;
-        s_img[e] = (uint8_t)lc_read(a, y0 - halo + r, x0 - halo + c);
+    // row r of the tile is the linear range from (y0 - halo + r, x0 - halo),
+    // clamped byte by byte into the allocation as in lc_read
+    for (int r = threadIdx.x >> 6; r < sh; r += 4) {
+        const int64_t row = (int64_t)(y0 - halo + r) * a.pitch + (x0 - halo);
+        for (int c = threadIdx.x & 63; c < sw; c += 64) {
+            int64_t off = row + c;
+            off = off < a.lo ? a.lo : (off >= a.hi ? a.hi - 1 : off);
+            s_img[r * sw + c] = a.in[off];
+        }
     }
     __syncthreads();
     const int x = x0 + (threadIdx.x & 63);

@@ -1231,11 +1231,8 @@ int launch_conv(const ConvArgs &a, hipStream_t s)
 {
     // Persistent: as many workgroups as are resident at once walk the 64 x 4
     // tiles (a workgroup beyond that would start only when one has finished).
-    static int resident = 0;
-    if (!resident) {
-        resident = resident_blocks(reinterpret_cast<const void *>(&conv_kernel<KIND>));
-        if (resident <= 0) return kHipError;
-    }
+    static const int resident = resident_blocks(reinterpret_cast<const void *>(&conv_kernel<KIND>));
+    if (resident <= 0) return kHipError;
     const int64_t ntiles = (int64_t)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.N;
     const int grid = (int)std::min<int64_t>(ntiles, resident);
     hipLaunchKernelGGL(conv_kernel<KIND>, dim3(grid), dim3(256), 0, s, a);
@@ -1248,12 +1245,9 @@ int launch_conv(const ConvArgs &a, hipStream_t s)
 int launch_fused(Net *net, const uint8_t *img, int64_t img_pitch, uint8_t *out, int64_t out_pitch,
                  int H, int W, float sigma_range, hipStream_t s)
 {
-    static int configured = 0;
-    if (!configured) {
-        BF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fused_infer_kernel),
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, FS_LDS));
-        configured = 1;
-    }
+    static const hipError_t configured = hipFuncSetAttribute(
+        reinterpret_cast<const void *>(fused_infer_kernel), hipFuncAttributeMaxDynamicSharedMemorySize, FS_LDS);
+    BF_HIP(configured);
     FusedArgs a{};
     a.img = img;
     a.img_pitch = img_pitch;
@@ -1322,14 +1316,10 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
     WgradArgs w = w0;
     w.partial = net->partial;
     const int64_t nseg = (int64_t)w.N * w.H * ((w.W + TX - 1) / TX);
-    static int resident = 0;
-    if (!resident) {
-        if constexpr (KIND == K_FIRST)
-            resident = resident_blocks(reinterpret_cast<const void *>(&wgrad_kernel<KIND>));
-        else
-            resident = resident_blocks(reinterpret_cast<const void *>(&wgrad32_kernel<KIND>));
-        if (resident <= 0) return kHipError;
-    }
+    static const int resident = resident_blocks(
+        KIND == K_FIRST ? reinterpret_cast<const void *>(&wgrad_kernel<K_FIRST>)
+                        : reinterpret_cast<const void *>(&wgrad32_kernel<KIND == K_FIRST ? K_HIDDEN : KIND>));
+    if (resident <= 0) return kHipError;
     const int blocks = (int)std::min<int64_t>(std::min(net->partial_blocks, resident), nseg);
     if constexpr (KIND == K_FIRST)
         hipLaunchKernelGGL(wgrad_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);

@@ -18,13 +18,20 @@ from opencuda_amd import workloads as wl
 pytestmark = pytest.mark.gpu
 
 
-def test_mixed_graph_subset(device):
+@pytest.mark.parametrize("sur_min_pixels", [0, 300_000])
+def test_mixed_graph_subset(device, sur_min_pixels):
     import torch
     items = [it for it in wl.c4(60) if it.pixels <= 1_600_000][:16]
     assert {it.radius for it in items} >= {15}
     net = sg.Surrogate(seed=1)
     net.load(sg.DEFAULT_WEIGHTS)
-    mx = mixed.Mixed(items, net, mode=oc.MODE_STD, streams=3)
+    mx = mixed.Mixed(items, net, mode=oc.MODE_STD, streams=3, sur_min_pixels=sur_min_pixels)
+    assert mx.sur_idx, "no image took the surrogate route"
+    # images below the size threshold take the exact batch, whatever their radius
+    small15 = [i for i, it in enumerate(items) if it.radius == 15 and it.pixels < sur_min_pixels]
+    assert set(small15) <= set(mx.ex_idx)
+    if sur_min_pixels:
+        assert small15, "the threshold routes no r = 15 image to the batch"
     mx.capture()
     eager = [t.cpu().numpy() for t in mx.tout]
     for t in mx.tout:
