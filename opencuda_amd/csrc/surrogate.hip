@@ -1232,7 +1232,10 @@ int launch_conv(const ConvArgs &a, hipStream_t s)
     // Persistent: as many workgroups as are resident at once walk the 64 x 4
     // tiles (a workgroup beyond that would start only when one has finished).
     static const int resident = resident_blocks(reinterpret_cast<const void *>(&conv_kernel<KIND>));
-    if (resident <= 0) return kHipError;
+    if (resident <= 0) {
+        set_error("conv_kernel: occupancy query failed");
+        return kHipError;
+    }
     const int64_t ntiles = (int64_t)((a.W + TX - 1) / TX) * ((a.H + TY - 1) / TY) * a.N;
     const int grid = (int)std::min<int64_t>(ntiles, resident);
     hipLaunchKernelGGL(conv_kernel<KIND>, dim3(grid), dim3(256), 0, s, a);
@@ -1319,7 +1322,10 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
     static const int resident = resident_blocks(
         KIND == K_FIRST ? reinterpret_cast<const void *>(&wgrad_kernel<K_FIRST>)
                         : reinterpret_cast<const void *>(&wgrad32_kernel<KIND == K_FIRST ? K_HIDDEN : KIND>));
-    if (resident <= 0) return kHipError;
+    if (resident <= 0) {
+        set_error("wgrad kernel: occupancy query failed");
+        return kHipError;
+    }
     const int blocks = (int)std::min<int64_t>(std::min(net->partial_blocks, resident), nseg);
     if constexpr (KIND == K_FIRST)
         hipLaunchKernelGGL(wgrad_kernel<KIND>, dim3(blocks), dim3(256), 0, s, w);
