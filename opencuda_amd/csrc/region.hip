@@ -383,17 +383,12 @@ __global__ void __launch_bounds__(256) cr_union_kernel(CrArgs a, int tiles_x, in
     if (!o && end && oa) cr_unite(a.label, idx, io + step);
 }
 
-__global__ void cr_compress_kernel(CrArgs a)
-{
-    const int64_t n = (int64_t)a.W * a.H;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        a.label[i] = cr_find(a.label, (int)i);
-}
-
-// Areas per root over the rows below a.covered: one wave per row walks it
-// in 64-pixel chunks.  Inside a chunk the lane that starts a run of equal
-// labels adds the run; the run that reaches the chunk's end is carried,
+// Path compression and areas in one pass: one wave per row walks it in
+// 64-pixel chunks, replaces every label by its root (the unions are all
+// done, so a root is the same whichever lane finds it; a walk through a
+// label another wave has already replaced reaches the same root), and for
+// the rows below a.covered adds the areas per root.  Inside a chunk the
+// lane that starts a run of equal roots adds the run; the run that reaches the chunk's end is carried,
 // with a wave-uniform label and count, into the next chunk.  A row's first
 // long run (>= 32 pixels) names its "hot" label, whose later runs are
 // summed in the wave and added once at the row's end: in a large region
@@ -404,8 +399,9 @@ __global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int y = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (y >= a.covered) return;                     // whole waves
-    const int *row = a.label + (int64_t)y * a.W;
+    if (y >= a.H) return;                           // whole waves
+    int *row = a.label + (int64_t)y * a.W;
+    const bool counted = y < a.covered;             // wave-uniform
     int cur = -1, cnt = 0;                          // carried run
     int hot = -1, hot_cnt = 0;                      // the row's hot label and its sum
     // a finished run of `lab` (wave-uniform): to the hot sum or one atomic
@@ -422,7 +418,10 @@ __global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
     };
     for (int x0 = 0; x0 < a.W; x0 += 64) {
         const int n = min(64, a.W - x0);            // pixels in this chunk
-        const int l = lane < n ? row[x0 + lane] : -1;
+        // path compression: the root, written back for the later passes
+        const int l = lane < n ? cr_find(a.label, y * a.W + x0 + lane) : -1;
+        if (lane < n) row[x0 + lane] = l;
+        if (!counted) continue;
         int prev = __shfl_up(l, 1);
         if (lane == 0) prev = cur;
         const bool start = lane < n && l != prev;
@@ -451,6 +450,7 @@ __global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
             cnt = n - last;
         }
     }
+    if (!counted) return;
     flush(cur, cnt);
     if (lane == 0 && hot >= 0) atomicAdd(a.area + hot, hot_cnt);
 }
@@ -643,9 +643,7 @@ int bf_connect_region(const bf_image_view *in, const bf_image_view *out, int32_t
         hipLaunchKernelGGL(cr_union_kernel, dim3((edges + 3) / 4), dim3(256), 0, s, a, tiles_x,
                            tiles_x * tiles_y);
     }
-    hipLaunchKernelGGL(cr_compress_kernel, dim3(flat), dim3(256), 0, s, a);
-    if (a.covered > 0)
-        hipLaunchKernelGGL(cr_area_kernel, dim3((a.covered + 3) / 4), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(cr_area_kernel, dim3((H + 3) / 4), dim3(256), 0, s, a);
     hipLaunchKernelGGL(cr_rowcount_kernel, dim3(H), dim3(64), 0, s, a);
     hipLaunchKernelGGL(cr_scan_kernel, dim3(1), dim3(1024), 0, s, a);
     hipLaunchKernelGGL(cr_number_kernel, dim3(H), dim3(64), 0, s, a);
