@@ -388,8 +388,9 @@ __global__ void __launch_bounds__(256) cr_union_kernel(CrArgs a, int tiles_x, in
 // done, so a root is the same whichever lane finds it; a walk through a
 // label another wave has already replaced reaches the same root), and for
 // the rows below a.covered adds the areas per root.  Inside a chunk the
-// lane that starts a run of equal roots adds the run; the run that reaches the chunk's end is carried,
-// with a wave-uniform label and count, into the next chunk.  A row's first
+// lane that starts a run of equal roots adds the run; the run that reaches
+// the chunk's end is carried, with a wave-uniform label and count, into the
+// next chunk.  A row's first
 // long run (>= 32 pixels) names its "hot" label, whose later runs are
 // summed in the wave and added once at the row's end: in a large region
 // broken by many small ones (here 1.26M runs on a 4096^2 image, half of them
@@ -420,7 +421,8 @@ __global__ void __launch_bounds__(256) cr_area_kernel(CrArgs a)
         const int n = min(64, a.W - x0);            // pixels in this chunk
         // path compression: the root, written back for the later passes
         const int l = lane < n ? cr_find(a.label, y * a.W + x0 + lane) : -1;
-        if (lane < n) row[x0 + lane] = l;
+        if (lane < n)
+            __hip_atomic_store(row + x0 + lane, l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (!counted) continue;
         int prev = __shfl_up(l, 1);
         if (lane == 0) prev = cur;

@@ -452,6 +452,8 @@ __device__ __forceinline__ void transpose8x8(const uint4 (&v)[8], uint4 (&out)[8
 // rows, K = pixels contiguous for the MFMA operands.
 constexpr int WG_PITCH = TX + 8;   // LDS row of 64 pixels + 8 pad (bank spread)
 
+// Register budget for 3 waves per SIMD (3 resident workgroups per CU; with
+// no bound the compiler's operand prefetch left 2, a budget for 4 spills).
 template <int KIND>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) wgrad32_kernel(WgradArgs a)
 {
