@@ -269,24 +269,28 @@ def _main() -> int:
                     help="fraction of crop offsets snapped to an image edge (trained unmasked)")
     ap.add_argument("--c3-frac", type=float, default=0.0,
                     help="fraction of training images with the 8192^2 benchmark image's settings")
+    ap.add_argument("--data-seed", type=int, default=2024, help="seed of the training images")
+    ap.add_argument("--crop-seed", type=int, default=7, help="seed of the crop sequence")
     args = ap.parse_args()
 
     def log(m):
         print(f"[surrogate] {m}", file=sys.stderr, flush=True)
 
     mode = MODE_STD if args.mode == "std" else MODE_REF
-    data = Dataset(args.images, args.size, mode=mode, log=log, c3_frac=args.c3_frac)
+    data = Dataset(args.images, args.size, mode=mode, log=log, c3_frac=args.c3_frac,
+                   seed=args.data_seed)
     net = Surrogate(args.seed)
     init_steps = 0
     if args.init:
         init_steps = int(net.load(args.init).get("steps", 0))
         log(f"starting from {args.init} ({init_steps} steps)")
-    hist = train(net, data, args.steps, args.batch, args.crop, args.lr, log=log,
-                 cosine=args.cosine, log_every=args.log_every, edge_frac=args.edge_frac)
+    hist = train(net, data, args.steps, args.batch, args.crop, args.lr, seed=args.crop_seed,
+                 log=log, cosine=args.cosine, log_every=args.log_every, edge_frac=args.edge_frac)
     net.save(args.out, history=np.array(hist, np.float64), mode=mode, radius=15,
              sigma_space=10.0, images=args.images, steps=init_steps + args.steps, batch=args.batch,
              crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine),
-             init_steps=init_steps, edge_frac=args.edge_frac, c3_frac=args.c3_frac)
+             init_steps=init_steps, edge_frac=args.edge_frac, c3_frac=args.c3_frac,
+             data_seed=args.data_seed, crop_seed=args.crop_seed)
     log(f"saved {args.out}")
     if args.eval:
         from . import workloads
