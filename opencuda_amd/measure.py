@@ -143,11 +143,11 @@ def c2(steps: int = 5, mode: int = MODE_REF, tune_table: str | None = None):
 
 
 def c4(count: int = 512, steps: int = 3, mode: int = MODE_STD, streams: int = 4,
-       weights: str | None = None, sur_min_pixels: int = 0):
+       weights: str | None = None, sur_min_pixels: int = 0, max_fallback: float | None = None):
     import torch
     from .mixed import measure
     out, mx = measure(count, steps, mode, log=_log, streams=streams, weights=weights,
-                      sur_min_pixels=sur_min_pixels)
+                      sur_min_pixels=sur_min_pixels, max_fallback=max_fallback)
     stream = torch.cuda.current_stream()
     jobs = [Job(view(mx.tin[i], it.roi), view(mx.tout[i]),
                 mx.luts[(it.sigma_space, it.sigma_range)].handle, it.radius, mode)
@@ -242,6 +242,9 @@ def _main() -> int:
     ap.add_argument("--weights", default=None, help="c4, c3-surrogate: surrogate weights (.npz)")
     ap.add_argument("--sur-min-pixels", type=int, default=0,
                     help="c4: r = 15 images smaller than this go to the exact batch")
+    ap.add_argument("--max-fallback", type=float, default=None,
+                    help="c4: calibrate, sending images whose fallback fraction exceeds this "
+                         "to the exact batch")
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
@@ -257,7 +260,7 @@ def _main() -> int:
     else:
         mode = MODE_REF if args.mode == "ref" else MODE_STD
         res = c4(args.count, args.steps or 3, mode, args.streams, args.weights,
-                 args.sur_min_pixels)
+                 args.sur_min_pixels, args.max_fallback)
     print(json.dumps(res), flush=True)
     return 0
 

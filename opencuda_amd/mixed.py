@@ -8,10 +8,15 @@ bf_batch launch.  The batch and the surrogate images are spread over
 several streams forked from the launching stream (each stream with its own
 network instance, whose check/fallback workspace it uses), so small images
 run side by side instead of one launch after another; the whole sequence is
-captured once into a graph and replayed.  Every image uses the same filter mode (STD by default: the mode
-the network approximates), so "exact" means the same thing for both groups.
+captured once into a graph and replayed.  Every image uses the same filter
+mode (STD by default: the mode the network approximates), so "exact" means
+the same thing for both groups.  Optional routing: `sur_min_pixels` sends
+smaller r >= 9 images to the batch; `max_fallback` filters each r >= 9
+image once through the network at construction and sends those whose check
+failed more than that fraction of their tiles to the batch (above about a
+third the network route costs more than the exact filter).
 
-    python -m opencuda_amd.mixed [--count 512] [--steps 3]
+    python -m opencuda_amd.mixed [--count 512] [--steps 3] [--max-fallback 0.33]
 """
 from __future__ import annotations
 
@@ -29,7 +34,8 @@ from .surrogate import DEFAULT_WEIGHTS, Surrogate
 
 class Mixed:
     def __init__(self, items, net: Surrogate, mode: int = MODE_STD, sur_min_radius: int = 9,
-                 tol: int = 2, streams: int = 4, log=None, sur_min_pixels: int = 0):
+                 tol: int = 2, streams: int = 4, log=None, sur_min_pixels: int = 0,
+                 max_fallback: float | None = None):
         import torch
         self.items = items
         self.mode = mode
@@ -61,6 +67,23 @@ class Mixed:
             (self.sur_idx if to_sur else self.ex_idx).append(i)
             if log and (i + 1) % 128 == 0:
                 log(f"{i + 1}/{len(items)} images uploaded ({time.time() - t0:.1f} s)")
+        # Calibration (max_fallback set): one eager network call per
+        # candidate.  An image whose sampled check sends more than
+        # max_fallback of its tiles to the exact filter costs more through
+        # the network than exactly (c3's costs put the break-even near 1/3),
+        # so it joins the exact batch instead.
+        self.calibration = {}
+        if max_fallback is not None:
+            keep = []
+            for i in self.sur_idx:
+                it = items[i]
+                total, fb = net.filter(view(self.tin[i], it.roi), view(self.tout[i]),
+                                       self.luts[(it.sigma_space, it.sigma_range)], it.radius,
+                                       mode, it.sigma_range, tol)
+                self.calibration[i] = (total, fb)
+                (keep if fb <= max_fallback * total else self.ex_idx).append(i)
+            self.sur_idx = keep
+            self.ex_idx.sort()
         jobs = [Job(view(self.tin[i], items[i].roi), view(self.tout[i]),
                     self.luts[(items[i].sigma_space, items[i].sigma_range)].handle,
                     items[i].radius, mode) for i in self.ex_idx]
@@ -130,12 +153,12 @@ class Mixed:
 
 
 def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None,
-            sur_min_pixels=0):
+            sur_min_pixels=0, max_fallback=None):
     import torch
     net = Surrogate(seed=1)
     net.load(weights or DEFAULT_WEIGHTS)
     mx = Mixed(wl.c4(count), net, mode=mode, streams=streams, log=log,
-               sur_min_pixels=sur_min_pixels)
+               sur_min_pixels=sur_min_pixels, max_fallback=max_fallback)
     if log:
         log(f"{len(mx.ex_idx)} exact images, {len(mx.sur_idx)} surrogate images, "
             f"{mx.pixels / 1e6:.1f} Mpx; capturing")
@@ -164,6 +187,7 @@ def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None
                     f"{'STD' if mode == MODE_STD else 'REF'} mode",
         "images": count, "exact_images": len(mx.ex_idx), "surrogate_images": len(mx.sur_idx),
         "streams": len(mx.nets), "sur_min_pixels": sur_min_pixels,
+        "max_fallback": max_fallback,
         "pixels": mx.pixels, "ms_per_graph": round(ms, 3),
         "mpix_s": round(mx.pixels / (ms / 1e3) / 1e6, 1),
         "tiles_checked": tiles // steps, "tiles_fallback": fb // steps,
@@ -181,13 +205,16 @@ def _main() -> int:
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--mode", choices=["std", "ref"], default="std")
     ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--sur-min-pixels", type=int, default=0)
+    ap.add_argument("--max-fallback", type=float, default=None)
     args = ap.parse_args()
 
     def log(m):
         print(f"[mixed] {m}", file=sys.stderr, flush=True)
 
     res, _ = measure(args.count, args.steps, MODE_STD if args.mode == "std" else MODE_REF, log=log,
-                     streams=args.streams)
+                     streams=args.streams, sur_min_pixels=args.sur_min_pixels,
+                     max_fallback=args.max_fallback)
     print(json.dumps(res), flush=True)
     return 0
 

@@ -5,7 +5,9 @@ sampled check + exact fallback for r = 15, captured in one graph.
 * surrogate images (spread over several streams in the graph): identical
   to a plain single-stream bf_sur_filter call, the tile count is every
   image's tiles, replays are identical to the eager run; error against the
-  exact filter and the FP64 oracle is reported."""
+  exact filter and the FP64 oracle is reported;
+* routing: by a minimum image size, and by a one-off calibration of each
+  image's fallback fraction (images above the threshold join the batch)."""
 import numpy as np
 import pytest
 
@@ -71,3 +73,36 @@ def test_mixed_graph_subset(device, sur_min_pixels):
     print(f"mixed subset: {len(mx.ex_idx)} exact + {len(mx.sur_idx)} surrogate images, "
           f"{fb}/{tiles} tiles fell back, max |err| vs exact {worst}, "
           f"max |err| vs FP64 on sampled rows {max(fp64):.3f}")
+
+
+def test_mixed_calibration_routes_by_fallback(device):
+    """max_fallback: each r >= 9 image is filtered once through the network
+    at construction; those whose check sent more than the threshold of their
+    tiles to the exact filter join the exact batch (and are then bit-exact
+    to the oracle), the others keep the network route, whose counts in the
+    graph equal the calibration's (the sampled check is deterministic)."""
+    items = [it for it in wl.c4(60) if it.pixels <= 1_600_000][:16]
+    net = sg.Surrogate(seed=1)
+    net.load(sg.DEFAULT_WEIGHTS)
+    thr = 0.35
+    mx = mixed.Mixed(items, net, mode=oc.MODE_STD, streams=2, max_fallback=thr)
+    cands = [i for i, it in enumerate(items) if it.radius >= 9]
+    assert cands and sorted(mx.calibration) == cands
+    for i, (total, fb) in mx.calibration.items():
+        assert total > 0 and 0 <= fb <= total
+        assert (i in mx.sur_idx) == (fb <= thr * total)
+    assert sorted(mx.sur_idx + mx.ex_idx) == list(range(len(items)))
+    mx.capture()
+    mx.reset_counts()
+    mx.replay()
+    tiles, fb = mx.counts()
+    assert tiles == sum(mx.calibration[i][0] for i in mx.sur_idx)
+    assert fb == sum(mx.calibration[i][1] for i in mx.sur_idx)
+    got = [t.cpu().numpy() for t in mx.tout]
+    for i in mx.ex_idx:
+        it = items[i]
+        exp = oracle.filter(mx.tin[i].cpu().numpy(), it.radius, it.sigma_space, it.sigma_range,
+                            mode=oracle.STD, roi=it.roi)
+        assert np.array_equal(got[i], exp), f"image {i} (r={it.radius})"
+    moved = [i for i in cands if i in mx.ex_idx]
+    print(f"calibration: {len(moved)} of {len(cands)} network candidates moved to the batch")
