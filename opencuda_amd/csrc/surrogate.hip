@@ -1246,7 +1246,7 @@ int launch_conv(const ConvArgs &a, hipStream_t s)
 }
 
 // Fused inference of one image (u8 in, u8 out): strips of FS_W columns x
-// bands of up to 256 rows, one workgroup of 10 waves each.
+// bands of up to 256 rows, one workgroup of FS_WAVES waves each.
 int launch_fused(Net *net, const uint8_t *img, int64_t img_pitch, uint8_t *out, int64_t out_pitch,
                  int H, int W, float sigma_range, hipStream_t s)
 {
