@@ -20,7 +20,41 @@
 namespace bfimpl {
 
 static thread_local std::string g_last_error;
-static std::atomic<int> g_next_lut_id{ 1 };
+
+// Table ids: unique among live tables and below 2^19, so the 19-bit id part
+// of a job's LDS key (make_job) never names two live tables.  Destroyed ids
+// are reused.
+constexpr int kMaxLutId = 0x7FFFF;
+struct LutIds {
+    std::mutex mu;
+    std::vector<int> free_ids;
+    int next = 1;
+};
+
+static LutIds &lut_ids()
+{
+    static LutIds *ids = new LutIds();
+    return *ids;
+}
+
+static int acquire_lut_id()
+{
+    LutIds &ids = lut_ids();
+    std::lock_guard<std::mutex> lock(ids.mu);
+    if (!ids.free_ids.empty()) {
+        const int id = ids.free_ids.back();
+        ids.free_ids.pop_back();
+        return id;
+    }
+    return ids.next <= kMaxLutId ? ids.next++ : -1;
+}
+
+static void release_lut_id(int id)
+{
+    LutIds &ids = lut_ids();
+    std::lock_guard<std::mutex> lock(ids.mu);
+    ids.free_ids.push_back(id);
+}
 
 void set_error(const std::string &msg) { g_last_error = msg; }
 
@@ -165,7 +199,7 @@ int make_job(const bf_image_view *in, const bf_image_view *out, const Lut *lut_b
     // LDS table identity: the same tables with another row count (REF has
     // the extra tap's row, and the zero row moves) or mode is another table.
     // (A fused table that fits in LDS has < 160 rows; a separate table is E only.)
-    job->lut_key = ((lut->id & 0x7FFFF) << 12) | ((sep ? 0 : nrows) << 1) | (sep ? 1 : 0);
+    job->lut_key = (lut->id << 12) | ((sep ? 0 : nrows) << 1) | (sep ? 1 : 0);
     job->tiles_x = (W + tc.tw - 1) / tc.tw;
     job->tile_begin = tile_begin;
     job->radius = radius;
@@ -200,7 +234,7 @@ bool ring_job(const Lut *L, int radius, int mode, int config, int halo, DevJob *
     job->lut_rows = rr + 1;
     // LDS key as make_job forms it: with one weight per q the per-q table
     // equals the per-weight one whenever their row counts agree
-    job->lut_key = ((L->id & 0x7FFFF) << 12) | (rr << 1);
+    job->lut_key = (L->id << 12) | (rr << 1);
     return true;
 }
 
@@ -209,7 +243,7 @@ bool ring64_job(const Lut *L, int radius, int mode, DevJob *job)
     return ring64_available(radius) && ring_job(L, radius, mode, 0, halo_class(radius), job);
 }
 
-static bool views_overlap(const bf_image_view *a, const bf_image_view *b)
+bool views_overlap(const bf_image_view *a, const bf_image_view *b)
 {
     const uint8_t *a0 = a->data, *a1 = a->data + (int64_t)(a->height - 1) * a->pitch + a->width;
     const uint8_t *b0 = b->data, *b1 = b->data + (int64_t)(b->height - 1) * b->pitch + b->width;
@@ -246,7 +280,12 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
     }
     auto *L = new (std::nothrow) LutFull();
     if (!L) return kOutOfMem;
-    L->id = g_next_lut_id.fetch_add(1);
+    L->id = acquire_lut_id();
+    if (L->id < 0) {
+        delete L;
+        set_error("too many live filter tables (2^19 - 1)");
+        return kOutOfMem;
+    }
     L->count = count;
     L->offsets.assign(offsets, offsets + 2 * (size_t)count);
     L->weights.assign(weights, weights + count);
@@ -350,6 +389,7 @@ int bf_lut_create(const int32_t *offsets, const float *weights, int32_t count, c
         hipFree(L->d_rowg);
         hipFree(L->d_range);
         hipFree(L->d_rowq);
+        release_lut_id(L->id);
         delete L;
         return hip_fail(e, "bf_lut_create");
     }
@@ -366,6 +406,7 @@ int bf_lut_destroy(bf_lut h)
     hipFree(L->d_rowq);
     hipFree(L->d_rowg);
     hipFree(L->d_range);
+    release_lut_id(L->id);
     delete L;
     return kOk;
 }

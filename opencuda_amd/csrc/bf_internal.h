@@ -132,6 +132,8 @@ int launch_ring(const DevJob &job, int radius, int mode, int ntiles, hipStream_t
 
 Lut *lut_from_handle(bf_lut h);
 int current_device();
+// True when the byte ranges two non-empty views span intersect.
+bool views_overlap(const bf_image_view *a, const bf_image_view *b);
 
 // Surrogate helpers used by bf_sur_filter.
 int exact_tiles(const DevJob &job, const int32_t *tile_list_dev, int ntiles_dev_max,

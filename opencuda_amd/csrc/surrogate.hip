@@ -1581,8 +1581,15 @@ static int infer_view(Net *net, const bf_image_view *in, const bf_image_view *ou
         return kNullPointer;
     }
     const int W = std::min(in->width, out->width), H = std::min(in->height, out->height);
-    if (W <= 0 || H <= 0) {
-        set_error("empty view");
+    if (W <= 0 || H <= 0 || in->pitch < in->width || out->pitch < out->width) {
+        set_error("empty view or pitch smaller than width");
+        return kInvalidData;
+    }
+    // The fused kernel reads a halo around every strip that neighbouring
+    // workgroups write, and the check and the fallback read the input again
+    // after inference: the input must survive the whole call.
+    if (views_overlap(in, out)) {
+        set_error("input and output views overlap");
         return kInvalidData;
     }
     *Wo = W;

@@ -14,8 +14,81 @@
 
 static void out(const char *k, long v) { std::printf("%s %ld\n", k, v); }
 
+// File-format subcommands (the test writes or parses the bytes itself):
+//   tplt-read PATH    read PATH, print "rc", "count", then "p x y" per point
+//                     and "w <bits>" per weight (IEEE bits in hex)
+//   tplt-write PATH   write a fixed 5-point template with negative offsets
+//   bmp-read PATH     read PATH, print "rc", "w", "h", the ROI and every row
+//                     as hex
+//   bmp-write PATH    write a 37 x 11 host image (pixel i = 7 i + 3) whose
+//                     ROI is (2, 3)-(30, 9): the whole image is written
+static int file_cmd(const char *cmd, const char *path)
+{
+    if (std::strcmp(cmd, "tplt-read") == 0) {
+        Template *t = NULL;
+        TemplateBasicOp::newTemplate(&t);
+        const int rc = TemplateBasicOp::readFromFile(path, t);
+        out("rc", rc);
+        if (rc == 0) {
+            out("count", (long)t->count);
+            for (size_t i = 0; i < t->count; i++)
+                std::printf("p %d %d\n", t->tplData[2 * i], t->tplData[2 * i + 1]);
+            for (size_t i = 0; i < t->count; i++) {
+                uint32_t bits;
+                std::memcpy(&bits, &ATTACHED_DATA(t)[i], 4);
+                std::printf("w %08x\n", bits);
+            }
+        }
+        TemplateBasicOp::deleteTemplate(t);
+        return 0;
+    }
+    if (std::strcmp(cmd, "tplt-write") == 0) {
+        Template *t = NULL;
+        TemplateBasicOp::newTemplate(&t);
+        TemplateBasicOp::makeAtHost(t, 5);
+        const int xy[10] = { 0, 0, -3, 7, 12, -1, -128, 65535, 2147483647, -2147483647 - 1 };
+        const float w[5] = { 1.0f, 0.5f, -2.25f, 1e-40f, 3.4e38f };
+        std::memcpy(t->tplData, xy, sizeof xy);
+        std::memcpy(ATTACHED_DATA(t), w, sizeof w);
+        out("rc", TemplateBasicOp::writeToFile(path, t));
+        TemplateBasicOp::deleteTemplate(t);
+        return 0;
+    }
+    if (std::strcmp(cmd, "bmp-read") == 0) {
+        Image *im = NULL;
+        ImageBasicOp::newImage(&im);
+        const int rc = ImageBasicOp::readFromFile(path, im);
+        out("rc", rc);
+        if (rc == 0) {
+            out("w", (long)im->width);
+            out("h", (long)im->height);
+            out("roi", (long)im->roiX1 * 1000000 + im->roiY1 * 10000 + im->roiX2 * 100 + im->roiY2);
+            for (size_t r = 0; r < im->height; r++) {
+                std::printf("row ");
+                for (size_t x = 0; x < im->width; x++) std::printf("%02x", im->imgData[r * im->width + x]);
+                std::printf("\n");
+            }
+        }
+        ImageBasicOp::deleteImage(im);
+        return 0;
+    }
+    if (std::strcmp(cmd, "bmp-write") == 0) {
+        Image *im = NULL;
+        ImageBasicOp::newImage(&im);
+        ImageBasicOp::makeAtHost(im, 37, 11);
+        for (int i = 0; i < 37 * 11; i++) im->imgData[i] = (unsigned char)(i * 7 + 3);
+        im->roiX1 = 2; im->roiY1 = 3; im->roiX2 = 30; im->roiY2 = 9;
+        out("rc", ImageBasicOp::writeToFile(path, im));
+        ImageBasicOp::deleteImage(im);
+        return 0;
+    }
+    std::fprintf(stderr, "unknown command %s\n", cmd);
+    return 2;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc > 2) return file_cmd(argv[1], argv[2]);
     const char *dir = argc > 1 ? argv[1] : ".";
     char path[4096];
 

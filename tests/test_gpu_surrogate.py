@@ -299,3 +299,29 @@ def test_trained_network_against_the_method_without_it(device):
     assert score["trained"] > score["identity"] + 0.2
     assert score["trained"] > score["untrained"] + 0.2
     assert mae["trained"] < 0.5 * mae["identity"]
+
+
+@pytest.mark.parametrize("entry", ["infer", "filter", "filter_async"])
+def test_overlapping_views_are_rejected(device, entry):
+    """In place (in == out) or partly overlapping views: the fused kernel
+    would overwrite halo pixels other workgroups still read, and the check
+    and fallback would read a modified input.  Rejected with INVALID_DATA
+    before any launch; the image is untouched."""
+    import torch
+    net = sg.Surrogate(seed=1)
+    lut = oc.Lut.from_sigmas(10.0, 30.0)
+    img = oc.synth_image(200, 150, 10, 8, 60, 10.0, 3)
+    t = torch.from_numpy(img).cuda()
+    before = t.clone()
+    for vin, vout in ((oc.view(t), oc.view(t)),
+                      (oc.view(t, (0, 0, 120, 100)), oc.view(t, (50, 40, 200, 150)))):
+        with pytest.raises(oc.BfError) as ei:
+            if entry == "infer":
+                net.infer(vin, vout, 30.0)
+            elif entry == "filter":
+                net.filter(vin, vout, lut, 15, oc.MODE_STD, 30.0)
+            else:
+                net.filter_async(vin, vout, lut, 15, oc.MODE_STD, 30.0)
+        assert ei.value.code == oc.INVALID_DATA
+    torch.cuda.synchronize()
+    assert torch.equal(t, before)
