@@ -23,6 +23,7 @@ import pytest
 import opencuda_amd as oc
 import oracle
 from opencuda_amd import surrogate as sg
+from tests._check import apply_fallback, expected_fallback
 
 pytestmark = pytest.mark.gpu
 
@@ -177,33 +178,6 @@ def test_no_fallback_is_pure_surrogate(device):
     assert torch.equal(a, b)
 
 
-def _hash32(x):
-    m = 0xFFFFFFFF
-    x ^= x >> 16
-    x = (x * 0x7feb352d) & m
-    x ^= x >> 15
-    x = (x * 0x846ca68b) & m
-    x ^= x >> 16
-    return x
-
-
-def _expected_fallback(sur, ex, tol, salt=0x5bd1e995):
-    """Tiles whose sampled |surrogate - exact| exceeds tol: one sample per
-    8x8 cell of each 64x64 tile at a hashed position."""
-    h, w = ex.shape
-    tx, ty = (w + 63) // 64, (h + 63) // 64
-    bad = np.zeros(tx * ty, bool)
-    for tile in range(tx * ty):
-        X0, Y0 = tile % tx * 64, tile // tx * 64
-        for cell in range(64):
-            hv = _hash32((tile * 64 + cell + salt) & 0xFFFFFFFF)
-            x = X0 + (cell & 7) * 8 + (hv & 7)
-            y = Y0 + (cell >> 3) * 8 + ((hv >> 3) & 7)
-            if x < w and y < h and abs(int(sur[y, x]) - int(ex[y, x])) > tol:
-                bad[tile] = True
-    return bad, tx
-
-
 @pytest.mark.parametrize("radius,mode,table", [
     (15, oc.MODE_STD, "gauss"),      # unrolled check + unrolled fallback
     (15, oc.MODE_REF, "gauss"),
@@ -232,16 +206,12 @@ def test_check_flags_exactly_the_sampled_tiles(device, radius, mode, table):
     total, fb = net.filter(oc.view(tin), oc.view(out), lut, radius, mode, 30.0, tol_lsb=2)
     torch.cuda.synchronize()
     s_np, e_np, o_np = sur.cpu().numpy(), ex.cpu().numpy(), out.cpu().numpy()
-    bad, tx = _expected_fallback(s_np, e_np, 2)
+    bad, tx = expected_fallback(s_np, e_np, 2)
     assert total == bad.size and fb == int(bad.sum())
     # the network approximates STD: in REF mode (4-row windows) every tile
     # may fail, in STD some must pass
     assert fb > 0 and (mode == oc.MODE_REF or fb < total)
-    exp = s_np.copy()
-    for tile in np.flatnonzero(bad):
-        X0, Y0 = tile % tx * 64, tile // tx * 64
-        exp[Y0:Y0 + 64, X0:X0 + 64] = e_np[Y0:Y0 + 64, X0:X0 + 64]
-    assert np.array_equal(o_np, exp)
+    assert np.array_equal(o_np, apply_fallback(s_np, e_np, bad, tx))
 
 
 @pytest.mark.parametrize("shape,roi,sr", [
