@@ -246,6 +246,24 @@ int bf_sur_filter_async(bf_sur sur, const bf_image_view *in, const bf_image_view
                         int32_t tol_lsb, int32_t *fallback_accum, int32_t *tiles_accum,
                         void *stream);
 
+/* bf_sur_filter_async without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under hipGraph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  The fallback tile count and list live in a file-scope device
+ * array of the library (one per device, looked up on the first call on that
+ * device and held until the process ends) instead of the network's
+ * workspace, so the call never grows a buffer.  Limit: 65536 tiles of
+ * 64x64, i.e. filtered images up to 16384 x 16384 pixels; larger ones are
+ * rejected with INVALID_DATA before anything is launched.  Calls on one
+ * device share that array: calls that overlap in time on one device (other
+ * streams, other threads) race and must be serialised by the caller.  The
+ * first call on a device should run outside graph capture (it looks up the
+ * array's address).  Output and counts as bf_sur_filter_async. */
+int bf_sur_filter_noalloc(bf_sur sur, const bf_image_view *in, const bf_image_view *out,
+                          bf_lut lut, int32_t radius, int32_t mode, float sigma_range,
+                          int32_t tol_lsb, int32_t *fallback_accum, int32_t *tiles_accum,
+                          void *stream);
+
 /* ---------------------------------------------------------------------
  * Misc.
  */

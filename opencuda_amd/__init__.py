@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
         "bf_sur_infer": ([P, VP, VP, F, P], ctypes.c_int),
         "bf_sur_filter": ([P, VP, VP, P, I32, I32, F, I32, P, P, P], ctypes.c_int),
         "bf_sur_filter_async": ([P, VP, VP, P, I32, I32, F, I32, P, P, P], ctypes.c_int),
+        "bf_sur_filter_noalloc": ([P, VP, VP, P, I32, I32, F, I32, P, P, P], ctypes.c_int),
         "bf_last_error": ([], ctypes.c_char_p),
         "bf_version": ([], ctypes.c_int),
         "bf_synth_image": ([P, I32, I32, I32, I32, I32, ctypes.c_double, ctypes.c_uint64],

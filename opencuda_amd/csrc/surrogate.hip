@@ -19,7 +19,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "bf_internal.h"
@@ -1605,27 +1607,51 @@ int bf_sur_infer(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, 
     return infer_view(net_of(hnd), in, out, sigma_range, (hipStream_t)stream, &W, &H);
 }
 
+// Where one call's fallback tile list and count live: the network's own
+// workspace (grown with hipMalloc as images grow), or caller-provided
+// device memory of `cap` tiles plus the count.
+struct FallbackWs {
+    int32_t *list = nullptr;
+    int32_t *count = nullptr;
+    int cap = 0;
+};
+
 static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *out, bf_lut lut,
                       int radius, int mode, float sigma_range, int tol_lsb, int32_t *accum,
-                      int32_t *tiles_accum, hipStream_t s)
+                      int32_t *tiles_accum, const FallbackWs *given, hipStream_t s)
 {
     // exact job for the check and the fallback (64 x 64 tiles = config 0)
     DevJob job;
     int rc = make_job(in, out, lut_from_handle(lut), radius, mode, 0, halo_class(radius), 0, &job,
                       nullptr);
     if (rc) return rc;
-    int W, H;
-    if ((rc = infer_view(net, in, out, sigma_range, s, &W, &H))) return rc;
+    const int W = std::min(in->width, out->width), H = std::min(in->height, out->height);
     const int tx = (W + 63) / 64, ty = (H + 63) / 64, nt = tx * ty;
-    if (nt > net->fb_tiles) {
-        hipFree(net->list);
-        net->list = nullptr;
-        net->fb_tiles = 0;
-        BF_HIP(hipMalloc(&net->list, nt * 4));
-        net->fb_tiles = nt;
+    FallbackWs ws;
+    if (given) {
+        if (nt > given->cap) {
+            set_error("the image has " + std::to_string(nt) + " 64x64 tiles; the workspace holds " +
+                      std::to_string(given->cap));
+            return kInvalidData;
+        }
+        ws = *given;
     }
-    if (!net->count) BF_HIP(hipMalloc(&net->count, 4));
-    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, net->count, 1);
+    int Wi, Hi;
+    if ((rc = infer_view(net, in, out, sigma_range, s, &Wi, &Hi))) return rc;
+    if (!given) {
+        if (nt > net->fb_tiles) {
+            hipFree(net->list);
+            net->list = nullptr;
+            net->fb_tiles = 0;
+            BF_HIP(hipMalloc(&net->list, nt * 4));
+            net->fb_tiles = nt;
+        }
+        if (!net->count) BF_HIP(hipMalloc(&net->count, 4));
+        ws.list = net->list;
+        ws.count = net->count;
+        ws.cap = net->fb_tiles;
+    }
+    hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, s, ws.count, 1);
     BF_HIP(hipGetLastError());
     // The canonical template at r = 9 or 15 runs the unrolled 64x64-tile
     // kernels for both the check and the fallback (same taps, order and
@@ -1637,8 +1663,8 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     ca.tiles_x = tx;
     ca.tiles_y = ty;
     ca.tol = tol_lsb;
-    ca.list = net->list;
-    ca.count = net->count;
+    ca.list = ws.list;
+    ca.count = ws.count;
     ca.accum = accum;
     ca.tiles_accum = tiles_accum;
     ca.salt = 0x5bd1e995u;
@@ -1647,8 +1673,46 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     ca.halo = halo_class(radius);
     if ((rc = launch_check(ca, ring ? radius : 0, nt, s))) return rc;
     // exact fallback over the listed tiles
-    if (ring) return ring64_tiles(job, radius, mode, net->list, net->count, nt, s);
-    return exact_tiles(job, net->list, nt, net->count, 0, halo_class(radius), s);
+    if (ring) return ring64_tiles(job, radius, mode, ws.list, ws.count, nt, s);
+    return exact_tiles(job, ws.list, nt, ws.count, 0, halo_class(radius), s);
+}
+
+// ------------------------------------------------- allocation-free variant
+//
+// bf_sur_filter_noalloc keeps its fallback tile count and list in this
+// file-scope device array instead of the network's workspace: word 0 is the
+// count, words 64.. the list.  Every device the library runs on holds its
+// own copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (the array's name is a
+// host-side stand-in, not a device address).
+constexpr int kScratchTiles = 65536;      // 64x64 tiles: up to 16384 x 16384 pixels
+constexpr int kScratchWords = 64 + kScratchTiles;
+static __device__ __align__(256) int32_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int32_t *g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device, with that device current), or nullptr with the error set.
+static int32_t *scratch_on_current_device()
+{
+    const int dev = current_device();
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("no current device, or device id beyond the scratch table");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            hip_fail(e, "hipGetSymbolAddress(scratch)");
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<int32_t *>(ptr);
+    }
+    return g_scratch_ptr[dev];
 }
 
 int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out, bf_lut lut,
@@ -1658,7 +1722,8 @@ int bf_sur_filter(bf_sur hnd, const bf_image_view *in, const bf_image_view *out,
     if (!hnd || !lut) return kNullPointer;
     Net *net = net_of(hnd);
     hipStream_t s = (hipStream_t)stream;
-    int rc = sur_filter(net, in, out, lut, radius, mode, sigma_range, tol_lsb, nullptr, nullptr, s);
+    int rc = sur_filter(net, in, out, lut, radius, mode, sigma_range, tol_lsb, nullptr, nullptr,
+                        nullptr, s);
     if (rc) return rc;
     const int W = std::min(in->width, out->width), H = std::min(in->height, out->height);
     if (tiles_total) *tiles_total = ((W + 63) / 64) * ((H + 63) / 64);
@@ -1677,7 +1742,23 @@ int bf_sur_filter_async(bf_sur hnd, const bf_image_view *in, const bf_image_view
 {
     if (!hnd || !lut) return kNullPointer;
     return sur_filter(net_of(hnd), in, out, lut, radius, mode, sigma_range, tol_lsb,
-                      fallback_accum, tiles_accum, (hipStream_t)stream);
+                      fallback_accum, tiles_accum, nullptr, (hipStream_t)stream);
+}
+
+int bf_sur_filter_noalloc(bf_sur hnd, const bf_image_view *in, const bf_image_view *out,
+                          bf_lut lut, int32_t radius, int32_t mode, float sigma_range,
+                          int32_t tol_lsb, int32_t *fallback_accum, int32_t *tiles_accum,
+                          void *stream)
+{
+    if (!hnd || !lut) return kNullPointer;
+    int32_t *base = scratch_on_current_device();
+    if (!base) return kHipError;
+    FallbackWs ws;
+    ws.count = base;
+    ws.list = base + 64;
+    ws.cap = kScratchTiles;
+    return sur_filter(net_of(hnd), in, out, lut, radius, mode, sigma_range, tol_lsb,
+                      fallback_accum, tiles_accum, &ws, (hipStream_t)stream);
 }
 
 }  // extern "C"
