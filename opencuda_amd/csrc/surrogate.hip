@@ -22,6 +22,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "bf_internal.h"
@@ -77,6 +78,35 @@ struct ConvArgs {
 __device__ __forceinline__ __bf16 to_bf(float v) { return (__bf16)v; }
 __device__ __forceinline__ float from_bf(__bf16 v) { return (float)v; }
 
+// Activation of a pre-activation sum (bias included): rounded to BF16, then
+// the ReLU on the BF16 bits -- a BF16 value is > 0 exactly when its bits,
+// read as int16, are > 0, so max(bits, 0) is the ReLU (-0 becomes +0).
+__device__ __forceinline__ __bf16 relu_bf(float v)
+{
+    const short b = __builtin_bit_cast(short, to_bf(v));
+    return __builtin_bit_cast(__bf16, (short)(b > 0 ? b : 0));
+}
+
+// Two activations packed in a dword (lo = first), with one packed max.
+__device__ __forceinline__ uint32_t act_pack2(float lo, float hi)
+{
+    typedef short i16x2_t __attribute__((ext_vector_type(2)));
+    typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+    const bf16x2_t b = { to_bf(lo), to_bf(hi) };
+    const i16x2_t v = __builtin_elementwise_max(__builtin_bit_cast(i16x2_t, b), (i16x2_t){ 0, 0 });
+    return __builtin_bit_cast(uint32_t, v);
+}
+
+// Eight BF16 values from two runs of four (the last layer's channel order
+// in a k-slot group: channels 4kq..4kq+3, then 16+4kq..16+4kq+3).
+__device__ __forceinline__ bf16x8 cat4(const __bf16 *p0, const __bf16 *p1)
+{
+    typedef __bf16 bf16x4_t __attribute__((ext_vector_type(4)));
+    const bf16x4_t a = *reinterpret_cast<const bf16x4_t *>(p0);
+    const bf16x4_t b = *reinterpret_cast<const bf16x4_t *>(p1);
+    return (bf16x8){ a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3] };
+}
+
 template <int KIND>
 struct KTraits {
     static constexpr int KS = (KIND == K_FIRST || KIND == K_DG_LAST) ? 1 : 9;   // 32-wide k-steps
@@ -113,13 +143,18 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
     const int64_t ntiles = (int64_t)tiles_x * tiles_y * a.N;
 
     // ---- B operand (weights) into registers, once per persistent workgroup
+    // (The last layer's k-slots of lane group kq hold channels 4kq..4kq+3 and
+    // 16+4kq..16+4kq+3: the order in which the fused kernel's layer-4 lanes
+    // hold their outputs.)
     bf16x8 bw[T::KS][T::NT];
 #pragma unroll
     for (int s = 0; s < T::KS; s++)
 #pragma unroll
-        for (int nt = 0; nt < T::NT; nt++)
-            bw[s][nt] = *reinterpret_cast<const bf16x8 *>(
-                a.wpk + ((size_t)(nt * 16 + (lane & 15)) * T::KTOT + s * 32 + 8 * (lane >> 4)));
+        for (int nt = 0; nt < T::NT; nt++) {
+            const __bf16 *w = a.wpk + ((size_t)(nt * 16 + (lane & 15)) * T::KTOT + s * 32);
+            bw[s][nt] = KIND == K_LAST ? cat4(w + 4 * (lane >> 4), w + 16 + 4 * (lane >> 4))
+                                       : *reinterpret_cast<const bf16x8 *>(w + 8 * (lane >> 4));
+        }
 
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int n = (int)(tile / ((int64_t)tiles_x * tiles_y));
@@ -160,14 +195,33 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
     }
     __syncthreads();
 
+    const int i16 = lane & 15, kq = lane >> 4;
+    // The forward layers start from their bias (the accumulator's initial
+    // value); the data-gradient kinds from zero.
     f32x4 acc[4][T::NT];
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
-        for (int nt = 0; nt < T::NT; nt++) acc[m][nt] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
+        for (int nt = 0; nt < T::NT; nt++) {
+            const float b = (KIND == K_FIRST || KIND == K_HIDDEN) ? a.bias[16 * nt + i16] : 0.0f;
+            acc[m][nt] = (f32x4){ b, b, b, b };
+        }
 
-    const int i16 = lane & 15, kq = lane >> 4;
-    if (T::STAGE32) {
+    if (KIND == K_LAST) {
+        // Per tap s the product of the 32 channels with W6[s] (zero
+        // accumulator), summed over the taps in order in FP32.
+#pragma unroll
+        for (int s = 0; s < 9; s++) {
+            const int dy = s / 3 - 1, dx = s % 3 - 1;
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                const int pix = (wave + 1 + dy) * (TX + 2) + (16 * m + i16 + 1 + dx);
+                const bf16x8 av = cat4(&s_in[pix * PITCH + 4 * kq], &s_in[pix * PITCH + 16 + 4 * kq]);
+                const f32x4 p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[s][0], (f32x4){ 0.f, 0.f, 0.f, 0.f }, 0, 0, 0);
+                acc[m][0] = s == 0 ? p : acc[m][0] + p;
+            }
+        }
+    } else if (T::STAGE32) {
 #pragma unroll
         for (int s = 0; s < 9; s++) {
             const int dy = s / 3 - 1, dx = s % 3 - 1;
@@ -248,12 +302,9 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
 #pragma unroll
                 for (int nt = 0; nt < T::NT; nt++) {
                     const int co = 16 * nt + i16;
-                    float v = acc[m][nt][r];
-                    if (KIND == K_FIRST || KIND == K_HIDDEN) {
-                        v += a.bias[co];
-                        v = v > 0.0f ? v : 0.0f;
-                    }
-                    s_out[(16 * m + 4 * kq + r) * PITCH + co] = to_bf(v);
+                    const float v = acc[m][nt][r];
+                    s_out[(16 * m + 4 * kq + r) * PITCH + co] =
+                        (KIND == K_FIRST || KIND == K_HIDDEN) ? relu_bf(v) : to_bf(v);
                 }
         __syncthreads();
         if (y < H) {
@@ -758,72 +809,66 @@ __global__ void pack_kernel(const float *params, PackTables pk)
 //
 // All six layers of the forward pass in one kernel, for inference: no
 // activation leaves the chip.  A workgroup owns a strip of FS_W output
-// columns and a band of rows and streams down the band.  At step t, layer l
-// (0..5) computes its row t - 2l; layer l's row r is read by layer l+1 at
-// steps r+2l+1 .. r+2l+3 (as its rows r-1 .. r+1), so each layer keeps a
-// ring of FS_RING = 4 rows in LDS.  Layer l's rows span FS_W + 2(5-l)
-// columns starting 5-l left of the strip: the context the layers above need.
+// columns and a band of rows and streams down the band two rows per step.
+// At step k, layer l (0..4) computes rows Y0 + 2k + 3(5-l) and the row
+// below (64 columns starting 5-l left of the strip: the context the layers
+// above need); layer l+1 reads them at steps k+1 and k+2 (as its rows
+// r-1 .. r+2), so each layer keeps a ring of FS_RING = 6 rows in LDS.
 //
 // Eight waves, two per SIMD.  Wave w runs hidden layer 1 + w/2 on M-tiles
-// 2(w%2) and 2(w%2)+1 (16 pixels each) with both 16-channel N-tiles, so each
-// A fragment read from LDS feeds two MFMAs.  Waves 0..3 also run M-tile w of
-// the last layer and store the output; waves 4..7 run M-tile w-4 of the
-// first layer (before their hidden MFMAs, so this VALU-heavy work overlaps
-// the other waves' MFMAs) and prefetch image rows.  Every wave keeps its B
-// operands in VGPRs for the whole band.  MFMAs per step: 45 for waves 0..3,
-// 38 for waves 4..7, i.e. 83 per SIMD.
-//
-// Image rows pass through a 64-row LDS ring, as u8 (the residual) and as
-// BF16 u/255 (the first layer's input): waves 4..7 load 16 rows into
-// registers 16 steps before the ring needs them (these waves issue no other
-// global memory operation, and the step barrier waits for LDS only, so the
-// loads overlap the MFMA work).
+// 2(w%2) and 2(w%2)+1 (16 pixels each), both rows, both 16-channel N-tiles:
+// 72 MFMAs per step.  The fragments of a layer's input rows r+1 and r+2 are
+// the input rows r'-1 and r' of the next step, so they stay in registers:
+// each step reads only two new input rows from LDS, and starts on the
+// fragments it already holds while the new ones arrive.  Waves 0..3 also
+// run M-tile w of the first layer (its input is a ring of image dwords
+// bf16(u/255) | bf16(sigma_r/100) << 16, zero outside the image); waves 6,7
+// (layer 4) feed their own BF16 outputs straight from registers into the
+// last layer, computed per pixel as D[t][x] = sum_ci W6[t][ci] * act[x][ci]
+// for the nine taps t (one MFMA with the taps as rows), written to a ring of
+// FP32 products; waves 4,5 sum each output pixel's nine products in tap
+// order, add the residual and store, and prefetch image rows 16 at a time
+// into registers eight steps before the ring needs them.
 //
 // The MFMAs compute the transposed product (weights as the A operand,
 // activations as B), so a lane ends with four consecutive channels of one
-// pixel and stores them with one 8-byte write.
-//
-// Activations are 64-byte NHWC pixels in ring rows whose pitch is a
-// multiple of 16 pixels; the 16-byte chunks of a pixel are permuted by a
-// function of its column mod 16 (fs_col), so the consecutive pixels one
-// ds_read_b128 gathers fall into distinct banks, and every per-lane part of
-// an address is the same at every step.
+// pixel and stores them with one 8-byte write.  Bias is the accumulator's
+// initial value; the epilogue rounds to BF16 and then applies the ReLU to
+// the BF16 bits.  Activations are 64-byte NHWC pixels in 64-pixel ring
+// rows; the 16-byte chunks of a pixel are permuted by a function of its
+// column mod 16 (fs_col), so the consecutive pixels one ds_read_b128
+// gathers fall into distinct banks.
 //
 // Operands, their order and every rounding are those of conv_kernel, so the
 // output equals the per-layer path's bit for bit.
 
-constexpr int FS_W = 54;                  // output columns per strip (the first layer's 64 = 4 M-tiles)
+constexpr int FS_W = 54;                  // output columns per strip
 constexpr int FS_WAVES = 8;
-constexpr int FS_RING = 4;                // rows per layer ring
+constexpr int FS_RING = 6;                // rows per activation / product ring
+constexpr int FS_ROWB = 64 * 64;          // bytes per activation ring row
+constexpr int FS_D_TAPS = 16;             // product rows per ring row (9 taps used)
+constexpr int FS_D_PITCH = 68;            // words per product row (64 columns + pad)
+constexpr int FS_D_ROWB = FS_D_TAPS * FS_D_PITCH * 4;
+constexpr int FS_D_OFF = 4 * FS_RING * FS_ROWB;                        // rings of layers 0..3 first
 constexpr int FS_IMG_ROWS = 64;           // image ring rows
+constexpr int FS_IMG_DUP = 3;             // ring rows 0..2 repeated after row 63
+constexpr int FS_IMG_PITCH = 80;          // image ring columns
+constexpr int FS_IMG_W = 66;              // image columns a strip reads
+constexpr int FS_IMGW_OFF = FS_D_OFF + FS_RING * FS_D_ROWB;            // dword ring (first layer)
+constexpr int FS_IMGU_OFF = FS_IMGW_OFF + (FS_IMG_ROWS + FS_IMG_DUP) * FS_IMG_PITCH * 4;  // u8 ring
+constexpr int FS_LUT_OFF = FS_IMGU_OFF + FS_IMG_ROWS * FS_IMG_PITCH;   // bf16(u / 255), u = 0..255
+constexpr int FS_CB_OFF = FS_LUT_OFF + 512;                           // biases of layers 0..4 (32 floats each)
+constexpr int FS_W0_OFF = FS_CB_OFF + 5 * 32 * 4;                      // first layer's A operand (64 lanes x 2 x 16 B)
+constexpr int FS_W6_OFF = FS_W0_OFF + 2 * 64 * 16;                     // last layer's A operand (64 lanes x 16 B)
+constexpr int FS_LDS = FS_W6_OFF + 64 * 16;
 constexpr int FS_CHUNK = 16;              // image rows per prefetch
-constexpr int FS_PRE = 19;                // image rows loaded before the first step
-constexpr int FS_PF = 5;                  // prefetched bytes per thread of waves 4..7
-#ifndef FS_AHEAD
-#define FS_AHEAD 2                        // k-steps the hidden layers' A reads run ahead
-#endif
-// Wave priorities (s_setprio) while a wave issues its hidden-layer MFMAs
-// (2) and builds the first layer's operands (1): the SIMD's arbiter then
-// prefers the MFMA stream over the other wave's VALU and LDS work (MI355X,
-// c3: 6.15 -> 5.87 ms; levels 1..3 and either section alone measured within
-// 2% of this; raising the last layer's section instead costs 1.5-3.5%).
-constexpr int FS_PRIO_HID = 2;
-constexpr int FS_PRIO_FIRST = 1;
-constexpr int fs_w(int l) { return FS_W + 2 * (NLAYERS - 1 - l); }     // 64 62 60 58 56 54
-constexpr int fs_pitch(int l) { return (fs_w(l) + 15) / 16 * 16; }     // 64 (pixels)
-constexpr int fs_off(int l) { return l == 0 ? 0 : fs_off(l - 1) + FS_RING * fs_pitch(l - 1) * 64; }
-constexpr int FS_IMG_W = fs_w(0) + 2;                                  // 66 image columns used
-constexpr int FS_IMG_PITCH = 80;          // image ring row pitch (elements)
-constexpr int FS_IMG_OFF = fs_off(NLAYERS - 1);                        // u8 image ring (the residual)
-constexpr int FS_IMGB_OFF = FS_IMG_OFF + FS_IMG_ROWS * FS_IMG_PITCH;  // BF16 u/255 ring (first layer)
-constexpr int FS_LUT_OFF = FS_IMGB_OFF + FS_IMG_ROWS * FS_IMG_PITCH * 2;  // bf16(u / 255), u = 0..255
-constexpr int FS_DUMP = FS_LUT_OFF + 512;                              // masked-off lanes' writes
-constexpr int FS_LDS = FS_DUMP + 64;
-constexpr int FS_CHUNK_BYTES = FS_CHUNK * FS_IMG_W;                    // 1056
-static_assert(fs_w(0) == 64 && fs_w(5) > 48, "four M-tiles per layer");
-static_assert(FS_IMG_PITCH >= FS_IMG_W, "image ring rows");
-static_assert(FS_CHUNK_BYTES <= FS_PF * 256, "a prefetch fits the registers of waves 4..7");
+constexpr int FS_PRE = 20;                // image rows loaded before the first step
+constexpr int FS_PF = 9;                  // prefetched bytes per thread of waves 4, 5
+constexpr int FS_KBEGIN = -10;            // first step: layer 0 computes rows Y0-5, Y0-4
+constexpr int FS_CHUNK_BYTES = FS_CHUNK * FS_IMG_W;
+static_assert(FS_CHUNK_BYTES <= FS_PF * 128, "a prefetch fits the registers of waves 4, 5");
 static_assert(FS_LDS <= 160 * 1024, "LDS");
+static_assert(FS_IMG_PITCH >= FS_IMG_W, "image ring rows");
 
 struct FusedArgs {
     const uint8_t *img;
@@ -834,36 +879,38 @@ struct FusedArgs {
     int band;                     // rows per band
     int strips;                   // strips per band row
     float sr;                     // sigma_range / 100
-    const __bf16 *fw[NLAYERS];    // packed forward B operands
+    const __bf16 *fw[NLAYERS];    // packed forward weight tables
     const float *bias[NLAYERS];
 };
 
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) bf16x8 fs_lds_b128;
+typedef __attribute__((address_space(3))) u32x2 fs_lds_u32x2;
+typedef __attribute__((address_space(3))) uint32_t fs_lds_u32;
+typedef __attribute__((address_space(3))) float fs_lds_f32;
 typedef __attribute__((address_space(3))) unsigned char fs_lds_u8;
 typedef __attribute__((address_space(3))) __bf16 fs_lds_bf;
-typedef __attribute__((address_space(3))) bf16x8 fs_lds_b128;
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(3))) bf16x4 fs_lds_b64;
 
-__device__ __forceinline__ bf16x8 fs_read_a(uint32_t addr) { return *(const fs_lds_b128 *)(size_t)addr; }
-__device__ __forceinline__ void fs_write_bf(uint32_t addr, __bf16 v) { *(fs_lds_bf *)(size_t)addr = v; }
-__device__ __forceinline__ void fs_write_b64(uint32_t addr, bf16x4 v) { *(fs_lds_b64 *)(size_t)addr = v; }
-__device__ __forceinline__ __bf16 fs_read_bf(uint32_t addr) { return *(const fs_lds_bf *)(size_t)addr; }
+__device__ __forceinline__ bf16x8 fs_read_frag(uint32_t addr) { return *(const fs_lds_b128 *)(size_t)addr; }
+__device__ __forceinline__ void fs_write_b64(uint32_t addr, u32x2 v) { *(fs_lds_u32x2 *)(size_t)addr = v; }
+__device__ __forceinline__ uint32_t fs_read_u32(uint32_t addr) { return *(const fs_lds_u32 *)(size_t)addr; }
+__device__ __forceinline__ void fs_write_u32(uint32_t addr, uint32_t v) { *(fs_lds_u32 *)(size_t)addr = v; }
+__device__ __forceinline__ float fs_read_f32(uint32_t addr) { return *(const fs_lds_f32 *)(size_t)addr; }
+__device__ __forceinline__ f32x4 fs_read_f4(uint32_t addr) { return *(const __attribute__((address_space(3))) f32x4 *)(size_t)addr; }
+__device__ __forceinline__ void fs_write_f32(uint32_t addr, float v) { *(fs_lds_f32 *)(size_t)addr = v; }
 __device__ __forceinline__ uint8_t fs_read_u8(uint32_t addr) { return *(const fs_lds_u8 *)(size_t)addr; }
 __device__ __forceinline__ void fs_write_u8(uint32_t addr, uint8_t v) { *(fs_lds_u8 *)(size_t)addr = v; }
+__device__ __forceinline__ void fs_write_bf(uint32_t addr, __bf16 v) { *(fs_lds_bf *)(size_t)addr = v; }
+__device__ __forceinline__ __bf16 fs_read_bf(uint32_t addr) { return *(const fs_lds_bf *)(size_t)addr; }
 
-// Bias, ReLU and BF16 rounding of the four consecutive channels a lane
-// holds after a transposed MFMA; zero outside the image.  (fmaxf and the
-// per-layer kernels' v > 0 ? v : 0 differ only in the sign of a zero, which
-// no later sum, bias or rounding can observe.)
-__device__ __forceinline__ bf16x4 fs_act4(f32x4 acc, const float *b, bool in)
+// Four consecutive channels of one pixel after a transposed MFMA (bias
+// included): BF16, ReLU, as two dwords.
+__device__ __forceinline__ u32x2 fs_act4(f32x4 acc)
 {
-    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-    bf16x4 o;
-#pragma unroll
-    for (int r = 0; r < 4; r++) o[r] = to_bf(fmaxf(acc[r] + b[r], 0.0f));
-    u32x2 bits = __builtin_bit_cast(u32x2, o);
-    bits = in ? bits : (u32x2){ 0u, 0u };
-    return __builtin_bit_cast(bf16x4, bits);
+    return (u32x2){ act_pack2(acc[0], acc[1]), act_pack2(acc[2], acc[3]) };
 }
 
 // Byte offset, within a ring row, of 16-byte chunk `chunk` of column x.
@@ -891,6 +938,12 @@ __device__ __forceinline__ uint8_t fs_load_img(const FusedArgs &a, int row, int 
     return a.img[(int64_t)row * a.img_pitch + col];
 }
 
+// Slot of a row in a 6-row ring (any row >= Y0 - 60; the steps touch rows
+// from Y0 - 21 on).
+__device__ __forceinline__ int fs_slot6(int row, int Y0) { return (row - Y0 + 60) % FS_RING; }
+
+template <int I> using fs_ic = std::integral_constant<int, I>;
+
 __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 {
     const int tid = threadIdx.x, lane = tid & 63;
@@ -900,258 +953,333 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const int x0 = strip * FS_W;
     const int Y0 = bandi * a.band;
     const int Y1 = min(Y0 + a.band, a.H);
+    const int KE = (Y1 - Y0 + 1) / 2;         // steps with output rows
     const int H = a.H, W = a.W;
-    const int img_base = Y0 - 6;              // image row held in ring slot 0
+    const int img_base = Y0 - 6;              // image row of ring row 0
     const int img_x = x0 - 6;                 // image column of ring column 0
-    const int t_begin = Y0 - 5, t_end = Y1 + 10;
-    const bool lastw = wave < 4;              // waves 0..3: last layer; 4..7: first layer + prefetch
-    const int ptid = tid - 256;               // prefetch thread index (waves 4..7)
+    const int ptid = tid - 256;               // prefetch thread (waves 4, 5)
 
-    // element index of image row `row` (>= img_base) in the image rings
-    auto img_idx = [&](int row) { return (uint32_t)(((row - img_base) & (FS_IMG_ROWS - 1)) * FS_IMG_PITCH); };
+    auto img_slot = [&](int row) { return (row - img_base) & (FS_IMG_ROWS - 1); };
 
-    // ---- B operands and biases
-    const int L = 1 + (wave >> 1);            // hidden layer
+    // ---- hidden layer L on M-tiles m0, m0 + 1: weights (A), bias (C)
+    const int L = 1 + (wave >> 1);
     const int m0 = 2 * (wave & 1);
     const __bf16 *fw = L == 1 ? a.fw[1] : (L == 2 ? a.fw[2] : (L == 3 ? a.fw[3] : a.fw[4]));
-    const float *bl = L == 1 ? a.bias[1] : (L == 2 ? a.bias[2] : (L == 3 ? a.bias[3] : a.bias[4]));
-    bf16x8 bw[9][2];
+    bf16x8 wA[9][2];
 #pragma unroll
     for (int s = 0; s < 9; s++)
 #pragma unroll
         for (int nt = 0; nt < 2; nt++)
-            bw[s][nt] = *reinterpret_cast<const bf16x8 *>(
-                fw + ((size_t)(nt * 16 + i16) * 288 + s * 32 + 8 * kq));
-    float hb[2][4];                           // bias of channels 16 nt + 4 kq + r
-#pragma unroll
-    for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) hb[nt][r] = bl[16 * nt + 4 * kq + r];
-    // bx: the last layer's nine k-steps (waves 0..3) or the first layer's two N-tiles (4..7)
-    bf16x8 bx[9];
-    float xb[2][4];
-    if (lastw) {
-        const __bf16 *fl = a.fw[NLAYERS - 1];
-#pragma unroll
-        for (int s = 0; s < 9; s++)
-            bx[s] = *reinterpret_cast<const bf16x8 *>(fl + ((size_t)i16 * 288 + s * 32 + 8 * kq));
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) xb[nt][r] = a.bias[NLAYERS - 1][0];
-    } else {
-        const __bf16 *f0 = a.fw[0];
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++)
-            bx[nt] = *reinterpret_cast<const bf16x8 *>(f0 + ((size_t)(nt * 16 + i16) * 32 + 8 * kq));
-#pragma unroll
-        for (int s = 2; s < 9; s++) bx[s] = bx[0];
-#pragma unroll
-        for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) xb[nt][r] = a.bias[0][16 * nt + 4 * kq + r];
-    }
-
-    // ---- per-lane, step-invariant address parts
-    // hidden layer L reads layer L-1: column c + 1 + dx, c = 16 m0 + i16 (+16 for the second M-tile)
+            wA[s][nt] = *reinterpret_cast<const bf16x8 *>(fw + ((size_t)(nt * 16 + i16) * 288 + s * 32 + 8 * kq));
+    // reads of layer L-1: column c + 1 + dx for c = 16 m0 + i16 (+ 16 for the second M-tile)
     uint32_t ha[3];
 #pragma unroll
     for (int d = 0; d < 3; d++) ha[d] = fs_col(16 * m0 + i16 + d, kq);
-    // hidden layer L writes column 16 m0 + i16 (+16 for the second M-tile),
-    // channels 16 nt + 4 kq .. +3: 8 bytes at byte 8 (kq & 1) of chunk 2 nt + kq / 2
+    // writes of layer L: channels 16 nt + 4 kq .. +3 of column 16 m0 + i16
     uint32_t hw[2];
 #pragma unroll
     for (int nt = 0; nt < 2; nt++) hw[nt] = fs_col(16 * m0 + i16, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
-    const int h_in_pitch = fs_pitch(0);                            // every ring row: 64 pixels
-    const int h_win = fs_w(0) - 2 * (L - 1), h_wout = h_win - 2;   // widths of layers L-1, L
-    const uint32_t h_in_off = L == 1 ? fs_off(0) : (L == 2 ? fs_off(1) : (L == 3 ? fs_off(2) : fs_off(3)));
-    const uint32_t h_out_off = L == 1 ? fs_off(1) : (L == 2 ? fs_off(2) : (L == 3 ? fs_off(3) : fs_off(4)));
-    const int h_cx = x0 - (5 - L);
-    // last layer (waves 0..3) reads layer 4 at column 16 wave + i16 + dx + 1
-    uint32_t la[3];
+    // image columns of layer L's two M-tiles; whole M-tiles inside the image need no mask
+    bool hcol[2];
+    bool hall[2];
 #pragma unroll
-    for (int d = 0; d < 3; d++) la[d] = fs_col(16 * (wave & 3) + i16 + d, kq);
-    // first layer (waves 4..7): this lane's four taps 4 kq + j (taps >= 9 are K padding)
-    int f_dy[4], f_dx[4];
-    bool f_tap[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int tap = 4 * kq + j;
-        f_tap[j] = tap < 9;
-        const int tp = tap < 9 ? tap : 4;
-        f_dy[j] = tp / 3 - 1;
-        f_dx[j] = tp % 3 - 1;
+    for (int mm = 0; mm < 2; mm++) {
+        const int c0 = x0 - (5 - L) + 16 * (m0 + mm);
+        hcol[mm] = c0 + i16 >= 0 && c0 + i16 < W;
+        hall[mm] = c0 >= 0 && c0 + 15 < W;
     }
+    const uint32_t h_in = (uint32_t)((L - 1) * FS_RING * FS_ROWB);
+    const uint32_t h_out = (uint32_t)(L * FS_RING * FS_ROWB);     // unused by layer 4
+    const int h_lo = Y0 - (5 - L), h_hi = Y1 + (5 - L);           // rows layer L must compute
 
-    // ---- tables and the first FS_PRE image rows
+    // ---- role data
+    const bool first_w = wave < 4;            // first layer, M-tile `wave`
+    // this lane's first-layer taps 4 kq + q; taps >= 9 are K padding whose
+    // weights are zero: they read the centre tap's (finite, >= 0) dword, so
+    // their products are +0, as the per-layer kernel's zero operands give
+    uint32_t ioff[4] = {};
+    bool fcol = false, fall = false;
+    if (first_w) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int t = 4 * kq + q < 9 ? 4 * kq + q : 4;
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            ioff[q] = (uint32_t)(((dy + 1) * FS_IMG_PITCH + 16 * wave + i16 + 1 + dx) * 4);
+        }
+        const int c0 = x0 - 5 + 16 * wave;
+        fcol = c0 + i16 >= 0 && c0 + i16 < W;
+        fall = c0 >= 0 && c0 + 15 < W;
+    }
+    const float b6 = a.bias[NLAYERS - 1][0];
+
+    // ---- tables, per-layer constants and the first FS_PRE image rows
     if (tid < 256) fs_write_bf(FS_LUT_OFF + 2 * tid, to_bf((float)tid / 255.0f));
+    if (tid < 5 * 32) fs_write_f32(FS_CB_OFF + 4 * tid, a.bias[tid >> 5][tid & 31]);
+    if (tid < 128) {
+        // first layer A operand of lane (kq, i16), N-tile nt: W0[co = 16 nt + i16][k = 8 kq ..]
+        const int nt = tid >> 6, l = tid & 63;
+        const u32x4 v = *reinterpret_cast<const u32x4 *>(a.fw[0] + ((size_t)(nt * 16 + (l & 15)) * 32 + 8 * (l >> 4)));
+        *(__attribute__((address_space(3))) u32x4 *)(size_t)(FS_W0_OFF + 16 * tid) = v;
+    } else if (tid < 192) {
+        // last layer A operand: A[t][k], tap t = i16 (< 9), k-slot (kq, q) = channel
+        // q < 4 ? 4 kq + q : 16 + 4 kq + q - 4
+        const int l = tid - 128, t = l & 15, q4 = 4 * (l >> 4);
+        bf16x8 v = {};
+        if (t < 9) v = cat4(a.fw[NLAYERS - 1] + t * 32 + q4, a.fw[NLAYERS - 1] + t * 32 + 16 + q4);
+        *(__attribute__((address_space(3))) bf16x8 *)(size_t)(FS_W6_OFF + 16 * l) = v;
+    }
+    const uint32_t sr_hi = (uint32_t)__builtin_bit_cast(uint16_t, to_bf(a.sr)) << 16;
     fs_step_barrier();
+    auto put_img = [&](int row, int cc, uint8_t u) {
+        const int sl = img_slot(row);
+        const int gx = img_x + cc;
+        const bool in = row >= 0 && row < H && gx >= 0 && gx < W;
+        const uint32_t wv = in ? ((uint32_t)__builtin_bit_cast(uint16_t, fs_read_bf(FS_LUT_OFF + 2 * u)) | sr_hi) : 0u;
+        fs_write_u8(FS_IMGU_OFF + sl * FS_IMG_PITCH + cc, u);
+        fs_write_u32(FS_IMGW_OFF + (sl * FS_IMG_PITCH + cc) * 4, wv);
+        if (sl < FS_IMG_DUP) fs_write_u32(FS_IMGW_OFF + ((sl + FS_IMG_ROWS) * FS_IMG_PITCH + cc) * 4, wv);
+    };
     for (int e = tid; e < FS_PRE * FS_IMG_W; e += FS_WAVES * 64) {
         const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-        const int row = t_begin - 1 + rr;
-        const uint8_t u = fs_load_img(a, row, img_x + cc);
-        const uint32_t ix = img_idx(row) + cc;
-        fs_write_u8(FS_IMG_OFF + ix, u);
-        fs_write_bf(FS_IMGB_OFF + 2 * ix, fs_read_bf(FS_LUT_OFF + 2 * u));
+        const int row = img_base + rr;
+        put_img(row, cc, fs_load_img(a, row, img_x + cc));
     }
     fs_step_barrier();
 
-    uint8_t pf[FS_PF];                        // prefetched image bytes (waves 4..7)
+    // hidden layer bias: the accumulators' initial value
+    f32x4 bias[2];
 #pragma unroll
-    for (int j = 0; j < FS_PF; j++) pf[j] = 0;
-    for (int t = t_begin; t < t_end; t++) {
-        const int phase = (t - t_begin) & (FS_CHUNK - 1);
-        if (!lastw && phase == 0) {
-            // rows [t + 18, t + 34): written at step t + 15, first read at step t + 17
-            const int r0 = t - 1 + FS_PRE;
+    for (int nt = 0; nt < 2; nt++)
 #pragma unroll
-            for (int j = 0; j < FS_PF; j++) {
-                const int e = ptid + 256 * j;
-                if (e < FS_CHUNK_BYTES) {
-                    const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-                    pf[j] = fs_load_img(a, r0 + rr, img_x + cc);
-                }
+        for (int r = 0; r < 4; r++) bias[nt][r] = a.bias[L][16 * nt + 4 * kq + r];
+
+    bf16x8 fr[3][3][2];                       // [row slot][dx][M-tile] fragments of layer L-1
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int mm = 0; mm < 2; mm++) fr[i][d][mm] = (bf16x8){};
+    uint32_t pf[(FS_PF + 3) / 4];             // prefetched image bytes, four per dword
+#pragma unroll
+    for (int j = 0; j < (FS_PF + 3) / 4; j++) pf[j] = 0;
+
+    // Every layer computes its two rows at every step, needed or not: a row
+    // outside a layer's needed range may hold garbage, but a needed row reads
+    // only needed rows of the layer below (its range is one row wider on each
+    // side), and those were written at earlier steps, so garbage never
+    // reaches an output.  Only the output stores are conditional.  The
+    // fragments a step carries in are the ones the previous step loaded.
+    //
+    // Role R_FIRST (waves 0..3): the first layer's M-tile `wave`; R_GATHER
+    // (waves 4, 5, layer 3): output rows and image prefetch; R_LAST (waves 6,
+    // 7, layer 4): the last layer's products.  MASKED: zero every activation
+    // outside the image (workgroups at an image border).
+    constexpr int R_FIRST = 0, R_GATHER = 1, R_LAST = 2;
+    auto step = [&](auto ROLE, auto MASKED, auto SA, auto SB, auto SC, int k) __attribute__((always_inline)) {
+        constexpr int RL = decltype(ROLE)::value;
+        constexpr bool MK = decltype(MASKED)::value;
+        constexpr int A = decltype(SA)::value, B = decltype(SB)::value, Cs = decltype(SC)::value;
+        const int r = Y0 + 2 * k + 3 * (5 - L);        // this layer's rows r, r + 1
+        auto rd = [&](int row, int d, int mm) {
+            return fs_read_frag(h_in + (uint32_t)(fs_slot6(row, Y0) * FS_ROWB) + ha[d] + mm * 1024);
+        };
+
+        // ---- loads: row r+1 (slot C) and the role's inputs
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int mm = 0; mm < 2; mm++) fr[Cs][d][mm] = rd(r + 1, d, mm);
+        const int r0 = Y0 + 2 * k + 15;                // first layer's rows (R_FIRST)
+        const int yg = Y0 + 2 * k + (wave - 4);        // output row (R_GATHER)
+        u32x4 fdw[2];
+        bf16x8 fw0[2];
+        f32x4 fb0[2];
+        float gv[9];
+        bf16x8 w6;
+        if constexpr (RL == R_FIRST) {
+#pragma unroll
+            for (int rw = 0; rw < 2; rw++) {
+                const uint32_t base = FS_IMGW_OFF + (uint32_t)(img_slot(r0 + rw - 1) * FS_IMG_PITCH * 4);
+#pragma unroll
+                for (int q = 0; q < 4; q++) fdw[rw][q] = fs_read_u32(base + ioff[q]);
             }
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                fw0[nt] = fs_read_frag(FS_W0_OFF + 1024 * nt + 16 * lane);
+                fb0[nt] = fs_read_f4(FS_CB_OFF + 64 * nt + 16 * kq);
+            }
+        } else if constexpr (RL == R_GATHER) {
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int dy = t / 3 - 1, dx = t % 3 - 1;
+                gv[t] = fs_read_f32(FS_D_OFF + (uint32_t)(fs_slot6(yg + dy, Y0) * FS_D_ROWB) +
+                                    (uint32_t)((t * FS_D_PITCH + lane + 1 + dx) * 4));
+            }
+        } else {
+            w6 = fs_read_frag(FS_W6_OFF + 16 * lane);
         }
 
-        if (!lastw) {
-            __builtin_amdgcn_s_setprio(FS_PRIO_FIRST);
-            // ---- first layer, row t (>= t_begin = Y0 - 5 always); waves 4..7 do
-            // this VALU-heavy work while waves 0..3 run their hidden MFMAs
-            const int r = t;
-            if (r < Y1 + 5) {
-                const bool row_in = r >= 0 && r < H;
-                const uint32_t wb = fs_off(0) + (uint32_t)((r & (FS_RING - 1)) * fs_pitch(0) * 64);
-                bool rowok[4];
-                uint32_t rb[4];
+        // epilogue of one row: BF16 + ReLU (zero outside the image), then the
+        // ring write, or for layer 4 the last layer's products
+        auto finish = [&](const f32x4 (&acc)[2][2], int row) __attribute__((always_inline)) {
+            const bool row_in = row >= 0 && row < H;
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int gy = r + f_dy[j];
-                    rowok[j] = f_tap[j] && gy >= 0 && gy < H;
-                    rb[j] = FS_IMGB_OFF + 2 * img_idx(gy);
-                }
-                const int c = 16 * (wave - 4) + i16;          // M-tile wave - 4 of 64 columns
-                bf16x8 av;
+            for (int mm = 0; mm < 2; mm++) {
+                u32x2 p[2];
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    const int col = c + 1 + f_dx[j];
-                    const int gx = img_x + col;
-                    const bool ok = rowok[j] && gx >= 0 && gx < W;
-                    const __bf16 v = fs_read_bf(rb[j] + 2 * col);
-                    av[2 * j] = ok ? v : to_bf(0.0f);
-                    av[2 * j + 1] = to_bf(ok ? a.sr : 0.0f);
+                for (int nt = 0; nt < 2; nt++) {
+                    p[nt] = fs_act4(acc[mm][nt]);
+                    if constexpr (MK) {
+                        const bool in = hcol[mm] && row_in;
+                        p[nt] = in ? p[nt] : (u32x2){ 0u, 0u };
+                    }
                 }
-                f32x4 acc0 = (f32x4){ 0.f, 0.f, 0.f, 0.f }, acc1 = acc0;
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[0], av, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[1], av, acc1, 0, 0, 0);
-                const int gx = x0 - 5 + c;
-                const bool in = row_in && gx >= 0 && gx < W;
+                if constexpr (RL != R_LAST) {
+                    const uint32_t wb = h_out + (uint32_t)(fs_slot6(row, Y0) * FS_ROWB) + mm * 1024;
+                    fs_write_b64(wb + hw[0], p[0]);
+                    fs_write_b64(wb + hw[1], p[1]);
+                } else {
+                    // D[t][x] = sum over the 32 channels of W6[t] * act[x]
+                    const bf16x8 bv = __builtin_bit_cast(bf16x8, (u32x4){ p[0][0], p[0][1], p[1][0], p[1][1] });
+                    const f32x4 dsum = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w6, bv, (f32x4){ 0.f, 0.f, 0.f, 0.f }, 0, 0, 0);
+                    const uint32_t db = FS_D_OFF + (uint32_t)(fs_slot6(row, Y0) * FS_D_ROWB) +
+                                        (uint32_t)(((4 * kq) * FS_D_PITCH + 16 * (m0 + mm) + i16) * 4);
+#pragma unroll
+                    for (int q = 0; q < 4; q++) fs_write_f32(db + q * FS_D_PITCH * 4, dsum[q]);
+                }
+            }
+        };
+
+        // ---- the MFMAs, ordered so that those on the carried rows (r-1 in
+        // slot A, r in slot B) come first: 36 of the 72 run before row r+1
+        // (slot C, loaded at the step's start) is needed.
+        f32x4 acc0[2][2], acc1[2][2];
+        auto mf = [&](f32x4 (&acc)[2][2], int s, const bf16x8 (&f)[3][2], int d) __attribute__((always_inline)) {
+#pragma unroll
+            for (int mm = 0; mm < 2; mm++)
 #pragma unroll
                 for (int nt = 0; nt < 2; nt++)
-                    fs_write_b64(wb + fs_col(c, 2 * nt + (kq >> 1)) + (kq & 1) * 8, fs_act4(nt ? acc1 : acc0, xb[nt], in));
-            }
-            if (phase == FS_CHUNK - 1) {
-                const int r0 = t - FS_CHUNK + FS_PRE;            // the rows prefetched at phase 0
+                    acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        wA[s][nt], f[d][mm], s == 0 ? bias[nt] : acc[mm][nt], 0, 0, 0);
+        };
+        // row r, dy = -1 (row r-1, slot A)
 #pragma unroll
-                for (int j = 0; j < FS_PF; j++) {
-                    const int e = ptid + 256 * j;
-                    if (e < FS_CHUNK_BYTES) {
-                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-                        const uint32_t ix = img_idx(r0 + rr) + cc;
-                        fs_write_u8(FS_IMG_OFF + ix, pf[j]);
-                        fs_write_bf(FS_IMGB_OFF + 2 * ix, fs_read_bf(FS_LUT_OFF + 2 * pf[j]));
-                    }
-                }
-            }
-            __builtin_amdgcn_s_setprio(0);
-        }
-        // ---- hidden layer L, row t - 2L
-        {
-            const int r = t - 2 * L;
-            if (r >= Y0 - (5 - L) && r < Y1 + (5 - L)) {
-                const bool row_in = r >= 0 && r < H;
-                f32x4 acc[2][2];
+        for (int d = 0; d < 3; d++) mf(acc0, d, fr[A], d);
+        __builtin_amdgcn_sched_barrier(0);
+        // row r+2 into slot A (row r-1 is consumed)
 #pragma unroll
-                for (int mm = 0; mm < 2; mm++)
+        for (int d = 0; d < 3; d++)
 #pragma unroll
-                    for (int nt = 0; nt < 2; nt++) acc[mm][nt] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
-                if (row_in) {
-                    __builtin_amdgcn_s_setprio(FS_PRIO_HID);
-                    // A fragments of k-step s + FS_AHEAD are read while the
-                    // MFMAs of k-step s run (the scheduling barriers keep the
-                    // reads ahead of the MFMAs that do not need them).
-                    uint32_t rbs[3];
+            for (int mm = 0; mm < 2; mm++) fr[A][d][mm] = rd(r + 2, d, mm);
+        // row r+1, dy = -1 and row r, dy = 0 (row r, slot B)
 #pragma unroll
-                    for (int d = 0; d < 3; d++)
-                        rbs[d] = h_in_off + (uint32_t)(((r + d - 1) & (FS_RING - 1)) * h_in_pitch * 64);
-                    bf16x8 av[FS_AHEAD + 1][2];
-#pragma unroll
-                    for (int s = 0; s < FS_AHEAD; s++)
-#pragma unroll
-                        for (int mm = 0; mm < 2; mm++)
-                            av[s][mm] = fs_read_a(rbs[s / 3] + ha[s % 3] + mm * 1024);
-#pragma unroll
-                    for (int s = 0; s < 9; s++) {
-                        if (s + FS_AHEAD < 9) {
-#pragma unroll
-                            for (int mm = 0; mm < 2; mm++)
-                                av[(s + FS_AHEAD) % (FS_AHEAD + 1)][mm] =
-                                    fs_read_a(rbs[(s + FS_AHEAD) / 3] + ha[(s + FS_AHEAD) % 3] + mm * 1024);
-                        }
-                        __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                        for (int mm = 0; mm < 2; mm++)
-#pragma unroll
-                            for (int nt = 0; nt < 2; nt++)
-                                acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    bw[s][nt], av[s % (FS_AHEAD + 1)][mm], acc[mm][nt], 0, 0, 0);
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-                    __builtin_amdgcn_s_setprio(0);
-                }
-                const uint32_t wb = h_out_off + (uint32_t)((r & (FS_RING - 1)) * 64 * 64);
-#pragma unroll
-                for (int mm = 0; mm < 2; mm++) {
-                    const int c = 16 * (m0 + mm) + i16;
-                    const int gx = h_cx + c;
-                    const bool in = row_in && gx >= 0 && gx < W;
-                    const bool keep = c < h_wout;
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++)
-                        fs_write_b64(keep ? wb + hw[nt] + mm * 1024 : FS_DUMP + 8 * (lane & 7),
-                                     fs_act4(acc[mm][nt], hb[nt], in));
-                }
-            }
+        for (int d = 0; d < 3; d++) {
+            mf(acc1, d, fr[B], d);
+            mf(acc0, 3 + d, fr[B], d);
         }
 
-        if (lastw) {
-            // ---- last layer, row t - 10, M-tile `wave`
-            const int r = t - 10;
-            if (r >= Y0 && r < Y1) {
-                f32x4 acc = (f32x4){ 0.f, 0.f, 0.f, 0.f };
-                uint32_t rbs[3];
+        // ---- the role's own work (independent of the rows above)
+        float gsum = 0.0f;
+        if constexpr (RL == R_FIRST) {
+            const int c0 = 16 * wave + i16;
 #pragma unroll
-                for (int d = 0; d < 3; d++)
-                    rbs[d] = fs_off(NLAYERS - 2) + (uint32_t)(((r + d - 1) & (FS_RING - 1)) * 64 * 64);
-                bf16x8 av[9];
+            for (int rw = 0; rw < 2; rw++) {
+                const int row = r0 + rw;
+                const bf16x8 av = __builtin_bit_cast(bf16x8, fdw[rw]);
+                u32x2 p[2];
 #pragma unroll
-                for (int s = 0; s < 9; s++) av[s] = fs_read_a(rbs[s / 3] + la[s % 3]);
-                __builtin_amdgcn_sched_barrier(0);
+                for (int nt = 0; nt < 2; nt++) {
+                    p[nt] = fs_act4(__builtin_amdgcn_mfma_f32_16x16x32_bf16(fw0[nt], av, fb0[nt], 0, 0, 0));
+                    if constexpr (MK) {
+                        const bool in = fcol && row >= 0 && row < H;
+                        p[nt] = in ? p[nt] : (u32x2){ 0u, 0u };
+                    }
+                }
+                const uint32_t wb = (uint32_t)(fs_slot6(row, Y0) * FS_ROWB);
+                fs_write_b64(wb + fs_col(c0, kq >> 1) + (kq & 1) * 8, p[0]);
+                fs_write_b64(wb + fs_col(c0, 2 + (kq >> 1)) + (kq & 1) * 8, p[1]);
+            }
+        } else if constexpr (RL == R_GATHER) {
+            gsum = gv[0];
 #pragma unroll
-                for (int s = 0; s < 9; s++)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bx[s], av[s], acc, 0, 0, 0);
-                // output channel 0 of pixel 16 wave + i16 is acc[0] of the lanes with kq = 0
-                const int c = 16 * wave + i16;
-                const int gx = x0 + c;
-                if (kq == 0 && c < FS_W && gx < W) {
-                    const uint8_t px = fs_read_u8(FS_IMG_OFF + img_idx(r) + c + 6);
-                    const float v = (float)px / 255.0f + (acc[0] + xb[0][0]);
-                    const float q = rintf(v * 255.0f);
-                    a.out[(int64_t)r * a.out_pitch + gx] = (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
+            for (int t = 1; t < 9; t++) gsum = gsum + gv[t];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+
+        // ---- row r, dy = 1 and row r+1, dy = 0 (row r+1, slot C); row r+1,
+        // dy = 1 (row r+2, slot A); row r's epilogue meanwhile
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            mf(acc0, 6 + d, fr[Cs], d);
+            mf(acc1, 3 + d, fr[Cs], d);
+        }
+#pragma unroll
+        for (int d = 0; d < 3; d++) mf(acc1, 6 + d, fr[A], d);
+        finish(acc0, r);
+        finish(acc1, r + 1);
+
+        // ---- output row (R_GATHER): residual + the summed products
+        if constexpr (RL == R_GATHER) {
+            const int gx = x0 + lane;
+            if (k >= 0 && yg < Y1 && lane < FS_W && gx < W) {
+                const uint8_t px = fs_read_u8(FS_IMGU_OFF + img_slot(yg) * FS_IMG_PITCH + lane + 6);
+                const float v = (float)px / 255.0f + (gsum + b6);
+                const float q = rintf(v * 255.0f);
+                a.out[(int64_t)yg * a.out_pitch + gx] = (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
+            }
+            const int phase = (k - FS_KBEGIN) & 7;
+            if (phase == 7) {
+                const int rp = img_base + FS_PRE + 2 * (k - 7 - FS_KBEGIN);   // the rows loaded at phase 0
+#pragma unroll
+                for (int j = 0; j < FS_PF; j++) {
+                    int e = ptid + 128 * j;
+                    asm volatile("" : "+v"(e));
+                    if (e < FS_CHUNK_BYTES) {
+                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
+                        put_img(rp + rr, cc, (uint8_t)(pf[j >> 2] >> (8 * (j & 3))));
+                    }
+                }
+            }
+            if (phase == 0) {
+                // image rows img_base + FS_PRE + 16 c .. + 15 (c = (k - FS_KBEGIN) / 8), written at phase 7
+                const int rp = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
+#pragma unroll
+                for (int j = 0; j < (FS_PF + 3) / 4; j++) pf[j] = 0;
+#pragma unroll
+                for (int j = 0; j < FS_PF; j++) {
+                    int e = ptid + 128 * j;
+                    asm volatile("" : "+v"(e));     // recomputed here, not held in registers across the loop
+                    if (e < FS_CHUNK_BYTES) {
+                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
+                        pf[j >> 2] |= (uint32_t)fs_load_img(a, rp + rr, img_x + cc) << (8 * (j & 3));
+                    }
                 }
             }
         }
         fs_step_barrier();
+    };
+
+    auto run = [&](auto ROLE, auto MASKED) __attribute__((always_inline)) {
+        for (int k = FS_KBEGIN; k < KE; k += 3) {
+            step(ROLE, MASKED, fs_ic<0>(), fs_ic<1>(), fs_ic<2>(), k);
+            if (k + 1 >= KE) break;
+            step(ROLE, MASKED, fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), k + 1);
+            if (k + 2 >= KE) break;
+            step(ROLE, MASKED, fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), k + 2);
+        }
+    };
+    // interior workgroup: every row and column a layer keeps lies in the image
+    const bool interior = x0 - 5 >= 0 && x0 + 58 < W && Y0 - 8 >= 0 && Y1 + 8 <= H;
+    const int role = wave < 4 ? R_FIRST : (wave < 6 ? R_GATHER : R_LAST);
+    if (interior) {
+        if (role == R_FIRST) run(fs_ic<R_FIRST>(), std::false_type());
+        else if (role == R_GATHER) run(fs_ic<R_GATHER>(), std::false_type());
+        else run(fs_ic<R_LAST>(), std::false_type());
+    } else {
+        if (role == R_FIRST) run(fs_ic<R_FIRST>(), std::true_type());
+        else if (role == R_GATHER) run(fs_ic<R_GATHER>(), std::true_type());
+        else run(fs_ic<R_LAST>(), std::true_type());
     }
 }
 
