@@ -52,6 +52,30 @@ def build() -> None:
     subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
 
 
+def build_native(out_path: str) -> str:
+    """Compile the same sources with -march=native into out_path (the CPU
+    baseline's timed build on the host that runs it); returns the flags."""
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"OUT={os.path.abspath(out_path)}"],
+                   check=True, capture_output=True, timeout=300)
+    return "-O3 -march=native -fopenmp -ffp-contract=off -fno-fast-math"
+
+
+SHIPPED_FLAGS = "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off -fno-fast-math"
+
+
+def use(path: str) -> None:
+    """Load the oracle from another build of the same sources (the native
+    CPU-baseline build); every function below then calls it."""
+    global _LIB_PATH, _lib
+    _LIB_PATH = os.path.abspath(path)
+    _lib = None
+
+
+def max_threads() -> int:
+    """omp_get_max_threads() of the loaded oracle build."""
+    return int(lib().orc_max_threads())
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
@@ -78,6 +102,8 @@ def lib() -> ctypes.CDLL:
                                      P, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_filter_f64.restype = ctypes.c_int
+        L.orc_max_threads.argtypes = []
+        L.orc_max_threads.restype = ctypes.c_int
         L.orc_set_mutation.argtypes = [ctypes.c_int]
         L.orc_set_mutation.restype = None
         I64 = ctypes.c_int64

@@ -363,3 +363,14 @@ int orc_filter_f64(const unsigned char *in, size_t inpitch, int inw, int inh,
     free(gw);
     return 0;
 }
+
+/* OpenMP's default thread count on this host (1 without OpenMP): recorded
+ * next to the CPU baseline's timings. */
+int orc_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
