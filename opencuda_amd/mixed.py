@@ -11,10 +11,12 @@ run side by side instead of one launch after another; the whole sequence is
 captured once into a graph and replayed.  Every image uses the same filter
 mode (STD by default: the mode the network approximates), so "exact" means
 the same thing for both groups.  Optional routing: `sur_min_pixels` sends
-smaller r >= 9 images to the batch; `max_fallback` filters each r >= 9
-image once through the network at construction and sends those whose check
-failed more than that fraction of their tiles to the batch (above about a
-third the network route costs more than the exact filter).
+smaller r >= 9 images to the batch; `max_fallback` filters every r >= 9
+image once through the network at construction (one timed multi-stream
+pass, `calibration_ms`) and sends those whose check failed more than that
+fraction of their tiles to the batch.  A single pass over fresh images then
+costs the calibration pass plus the exact batch (`one_pass_ms` in
+`measure`), not the graph replay alone.
 
     python -m opencuda_amd.mixed [--count 512] [--steps 3] [--max-fallback 0.33]
 """
@@ -67,23 +69,13 @@ class Mixed:
             (self.sur_idx if to_sur else self.ex_idx).append(i)
             if log and (i + 1) % 128 == 0:
                 log(f"{i + 1}/{len(items)} images uploaded ({time.time() - t0:.1f} s)")
-        # Calibration (max_fallback set): one eager network call per
-        # candidate.  An image whose sampled check sends more than
-        # max_fallback of its tiles to the exact filter costs more through
-        # the network than exactly (c3's costs put the break-even near 1/3),
-        # so it joins the exact batch instead.
+        # Calibration (max_fallback set): an image whose sampled check sends
+        # more than max_fallback of its tiles to the exact filter costs more
+        # through the network than exactly, so it joins the exact batch.
         self.calibration = {}
+        self.calibration_ms = None
         if max_fallback is not None:
-            keep = []
-            for i in self.sur_idx:
-                it = items[i]
-                total, fb = net.filter(view(self.tin[i], it.roi), view(self.tout[i]),
-                                       self.luts[(it.sigma_space, it.sigma_range)], it.radius,
-                                       mode, it.sigma_range, tol)
-                self.calibration[i] = (total, fb)
-                (keep if fb <= max_fallback * total else self.ex_idx).append(i)
-            self.sur_idx = keep
-            self.ex_idx.sort()
+            self._calibrate(max_fallback)
         jobs = [Job(view(self.tin[i], items[i].roi), view(self.tout[i]),
                     self.luts[(items[i].sigma_space, items[i].sigma_range)].handle,
                     items[i].radius, mode) for i in self.ex_idx]
@@ -93,6 +85,48 @@ class Mixed:
         self.graph = None
         self.pixels = sum(it.pixels for it in items)
         self.sur_pixels = sum(items[i].pixels for i in self.sur_idx)
+
+    def _calibrate(self, max_fallback: float) -> None:
+        """One pass of every network candidate through the network (check and
+        fallback included, spread over the streams like the graph), each
+        image's fallback count into its own device counter; then the images
+        whose check failed more than max_fallback of their tiles join the
+        exact batch.  The pass already leaves the final output of the images
+        that stay on the network.  calibration_ms: its GPU time (events on
+        the launching stream around the fork/join), host routing excluded."""
+        import torch
+        cand = sorted(self.sur_idx, key=lambda i: -self.items[i].pixels)
+        fb = torch.zeros(max(1, len(cand)), dtype=torch.int32, device="cuda")
+        tiles = torch.zeros_like(fb)
+        main = torch.cuda.current_stream()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        # twice: the first pass grows each network's workspace to its largest
+        # image; the second is timed (same counts: the check is deterministic)
+        for rep in range(2):
+            fb.zero_()
+            tiles.zero_()
+            e0.record(main)
+            for st in self.streams:
+                st.wait_stream(main)
+            for k, i in enumerate(cand):
+                it = self.items[i]
+                j = k % len(self.nets)
+                self.nets[j].filter_async(view(self.tin[i], it.roi), view(self.tout[i]),
+                                          self.luts[(it.sigma_space, it.sigma_range)], it.radius,
+                                          self.mode, it.sigma_range, self.tol, fb[k:k + 1],
+                                          tiles[k:k + 1], self.streams[j])
+            for st in self.streams:
+                main.wait_stream(st)
+            e1.record(main)
+            torch.cuda.synchronize()
+        self.calibration_ms = e0.elapsed_time(e1)
+        fbh, th = fb.cpu().tolist(), tiles.cpu().tolist()
+        keep = []
+        for k, i in enumerate(cand):
+            self.calibration[i] = (th[k], fbh[k])
+            (keep if fbh[k] <= max_fallback * th[k] else self.ex_idx).append(i)
+        self.sur_idx = sorted(keep)
+        self.ex_idx.sort()
 
     def _sur_order(self):
         # largest first, so the network's workspace reaches its final size
@@ -182,6 +216,25 @@ def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None
         errs.append((int(d.max()), float(d.sum()), d.size))
         within.append(int((d <= 2).sum()))
     npx = sum(e[2] for e in errs)
+    # A single pass over fresh images: with routing by radius the graph is
+    # that pass; with calibrated routing it is the calibration pass (which
+    # already produced the final output of the images kept on the network)
+    # plus the exact batch over every other image.
+    if max_fallback is None:
+        one_pass = ms
+    else:
+        bms = 0.0
+        if mx.batch is not None:
+            stream = torch.cuda.current_stream()
+            mx.batch.run(stream)
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(steps):
+                mx.batch.run(stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            bms = e0.elapsed_time(e1) / steps
+        one_pass = mx.calibration_ms + bms
     out = {
         "workload": f"c4: {count} images 128..4096 per side, r in {{3,7,15}}, "
                     f"{'STD' if mode == MODE_STD else 'REF'} mode",
@@ -189,6 +242,9 @@ def measure(count=512, steps=3, mode=MODE_STD, log=None, streams=4, weights=None
         "streams": len(mx.nets), "sur_min_pixels": sur_min_pixels,
         "max_fallback": max_fallback,
         "pixels": mx.pixels, "ms_per_graph": round(ms, 3),
+        "one_pass_ms": round(one_pass, 3),
+        "one_pass_mpix_s": round(mx.pixels / (one_pass / 1e3) / 1e6, 1),
+        "calibration_ms": None if mx.calibration_ms is None else round(mx.calibration_ms, 3),
         "mpix_s": round(mx.pixels / (ms / 1e3) / 1e6, 1),
         "tiles_checked": tiles // steps, "tiles_fallback": fb // steps,
         "fallback_fraction": round(fb / max(tiles, 1), 4),
