@@ -123,7 +123,9 @@ typedef struct bf_job {
 typedef struct bf_batch_s *bf_batch;
 
 /* Tile configuration ids accepted by bf_batch_create / bf_exact_config.
- * bf_tune_count() reports how many exist; bf_tune_describe() names one. */
+ * bf_tune_count() reports how many exist; bf_tune_describe() names one:
+ * tile width and height in pixels, threads per workgroup, and the adjacent
+ * columns each lane filters (1 or 2; each lane owns 4 rows of them). */
 int bf_tune_count(void);
 int bf_tune_describe(int32_t config, int32_t *tile_w, int32_t *tile_h,
                      int32_t *threads, int32_t *cols_per_lane);
@@ -142,6 +144,9 @@ int bf_batch_run(bf_batch batch, void *stream);
 int bf_batch_run_graph(bf_batch batch, void *stream);
 /* Pixels the batch filters (sum of min(in,out) areas). */
 int64_t bf_batch_pixels(bf_batch batch);
+/* The tile configuration the batch runs with (the one asked for, or the
+ * one bf_batch_create chose for config < 0). */
+int bf_batch_config(bf_batch batch);
 /* 1 when every job uses the canonical GAUSS-factory template at a radius
  * with an unrolled kernel (3, 5, 7, 9, 15) and the batch runs those
  * kernels (tile configs 0 and 1), 0 when it runs the tap-list kernel,
@@ -149,10 +154,12 @@ int64_t bf_batch_pixels(bf_batch batch);
 int bf_batch_unrolled(bf_batch batch);
 
 /* Autotuner (no reference counterpart).  bf_tune_radius times every tile
- * configuration for (radius, mode) on the current device -- a synthetic
- * width x height image, sigma_space = radius/1.5, sigma_range = 30, `reps`
- * timed launches each after one warm-up -- and installs the fastest in a
- * process-wide table.  bf_exact and bf_batch_create (config < 0) use that
+ * configuration (tile shape and columns per lane, bf_tune_describe) for
+ * (radius, mode) on the current device -- a synthetic width x height image,
+ * sigma_space = radius/1.5, sigma_range = 30, run as a one-job batch with
+ * that configuration (so the unrolled kernels are timed where a batch would
+ * use them), `reps` timed launches after one warm-up -- and installs the
+ * fastest in a process-wide table.  bf_exact and bf_batch_create (config < 0) use that
  * table: a single image takes the tuned configuration of its radius; a
  * batch whose radii are all tuned takes the configuration with the least
  * estimated total time (sum over jobs of pixels x measured cost).

@@ -92,6 +92,7 @@ def lib() -> ctypes.CDLL:
         "bf_batch_run_graph": ([P, P], ctypes.c_int),
         "bf_batch_pixels": ([P], I64),
         "bf_batch_unrolled": ([P], ctypes.c_int),
+        "bf_batch_config": ([P], ctypes.c_int),
         "bf_sur_param_count": ([], I64),
         "bf_sur_create": ([ctypes.c_uint64, ctypes.POINTER(P)], ctypes.c_int),
         "bf_sur_destroy": ([P], ctypes.c_int),
@@ -234,13 +235,14 @@ def exact_inplace(v: View, lut: Lut, radius: int, mode: int = MODE_REF, repeat: 
 
 
 def tune_configs():
-    """[(id, tile_w, tile_h, threads)] of the exact kernel's tile shapes."""
+    """[(id, tile_w, tile_h, threads, cols_per_lane)] of the exact kernel's
+    tile configurations."""
     out = []
     for c in range(lib().bf_tune_count()):
         tw, th, nt, cpl = (ctypes.c_int32() for _ in range(4))
         _check(lib().bf_tune_describe(c, ctypes.byref(tw), ctypes.byref(th), ctypes.byref(nt),
                                       ctypes.byref(cpl)), "bf_tune_describe")
-        out.append((c, tw.value, th.value, nt.value))
+        out.append((c, tw.value, th.value, nt.value, cpl.value))
     return out
 
 
@@ -255,6 +257,7 @@ class Batch:
         self.handle = h
         self.pixels = int(lib().bf_batch_pixels(h))
         self.unrolled = bool(_check(lib().bf_batch_unrolled(h), "bf_batch_unrolled"))
+        self.config = int(_check(lib().bf_batch_config(h), "bf_batch_config"))
 
     def run(self, stream=None):
         _check(lib().bf_batch_run(self.handle, _stream(stream)), "bf_batch_run")

@@ -1,10 +1,12 @@
-"""Autotuner for the exact filter's tile shape (bf_tune_radius).
+"""Autotuner for the exact filter's tile configuration: tile shape and
+columns per lane (bf_tune_radius, bf_tune_describe).
 
 For each (radius, mode) the library times every tile configuration on the
 device and keeps the fastest in a process-wide table that bf_exact and
 bf_batch_create consult (a batch picks the configuration with the least
 estimated total time over its jobs).  Tables can be saved to JSON and
-installed again in a later process (bf_tune_set).
+installed again in a later process (bf_tune_set).  `tune_batch` times a
+given batch itself with every configuration.
 
     python -m opencuda_amd.autotune --out opencuda_amd/data/tune_gfx950.json
 """
@@ -31,7 +33,7 @@ def _nconf() -> int:
     return int(_lib().bf_tune_count())
 
 
-def tune(radius: int, mode: int, width: int = 2048, height: int = 1024, reps: int = 3):
+def tune(radius: int, mode: int, width: int = 4096, height: int = 2048, reps: int = 3):
     """Time every configuration; returns (best_config, ms_per_mpix list)."""
     from . import _check
     cost = np.zeros(_nconf(), np.float32)
@@ -58,7 +60,36 @@ def reset() -> None:
     _lib().bf_tune_reset()
 
 
-def tune_all(radii=(3, 5, 7, 9, 15), modes=(0, 1), width=2048, height=1024, reps=3, log=None):
+def tune_batch(jobs, reps: int = 3, stream=None):
+    """Time a batch of bf_job entries with every tile configuration that
+    can run it (each as its own bf_batch, one warm-up run, `reps` timed
+    runs); returns (best_config, [ms per run or None per configuration]).
+    The per-radius table predicts a batch from single images; this measures
+    the batch itself -- its mix of image sizes and partial tiles."""
+    import torch
+    from . import Batch, BfError
+    s = stream if stream is not None else torch.cuda.current_stream()
+    times = []
+    for c in range(_nconf()):
+        try:
+            b = Batch(jobs, c)
+        except BfError:
+            times.append(None)
+            continue
+        b.run(s)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(reps):
+            b.run(s)
+        e1.record(s)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / reps)
+        b.close()
+    ok = [(t, c) for c, t in enumerate(times) if t is not None]
+    return (min(ok)[1] if ok else -1), times
+
+
+def tune_all(radii=(3, 5, 7, 9, 15), modes=(0, 1), width=4096, height=2048, reps=3, log=None):
     table = {}
     for mode in modes:
         for r in radii:
@@ -96,8 +127,8 @@ def _main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--radii", default="1,2,3,4,5,6,7,8,9,10,11,12,13,14,15,16,17,18,19")
-    ap.add_argument("--width", type=int, default=2048)
-    ap.add_argument("--height", type=int, default=1024)
+    ap.add_argument("--width", type=int, default=4096)
+    ap.add_argument("--height", type=int, default=2048)
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args()
     radii = [int(r) for r in args.radii.split(",")]

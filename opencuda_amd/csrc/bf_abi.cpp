@@ -429,7 +429,7 @@ int bf_tune_describe(int32_t config, int32_t *tile_w, int32_t *tile_h, int32_t *
     if (tile_w) *tile_w = c.tw;
     if (tile_h) *tile_h = 4 * c.tgy;
     if (threads) *threads = c.threads;
-    if (cols_per_lane) *cols_per_lane = 1;
+    if (cols_per_lane) *cols_per_lane = c.cpl;
     return kOk;
 }
 
@@ -531,17 +531,26 @@ int bf_tune_radius(int32_t radius, int32_t mode, int32_t width, int32_t height, 
     if (e == hipSuccess) e = hipEventCreate(&e1);
     if (e != hipSuccess) rc = hip_fail(e, "bf_tune_radius");
     bf_image_view vin{ din, width, height, width }, vout{ dout, width, height, width };
+    // Each configuration is timed as a one-job batch: the launch a batch
+    // with that configuration makes (the unrolled kernels where they apply,
+    // configurations 0 and 1 at r = 3, 5, 7, 9, 15; the tap-list kernel
+    // otherwise), which is what the table's costs steer.
+    const bf_job job{ vin, vout, lut, radius, mode };
     for (int c = 0; c < nc && rc == kOk; c++) {
-        if (bf_exact_config(&vin, &vout, lut, radius, mode, c, nullptr) != kOk) continue;
-        hipEventRecord(e0, nullptr);
-        int r2 = kOk;
-        for (int k = 0; k < reps && r2 == kOk; k++)
-            r2 = bf_exact_config(&vin, &vout, lut, radius, mode, c, nullptr);
-        hipEventRecord(e1, nullptr);
+        bf_batch b = nullptr;
+        if (bf_batch_create(&job, 1, c, &b) != kOk) continue;
+        int r2 = bf_batch_run(b, nullptr);
+        if (r2 == kOk) {
+            hipEventRecord(e0, nullptr);
+            for (int k = 0; k < reps && r2 == kOk; k++) r2 = bf_batch_run(b, nullptr);
+            hipEventRecord(e1, nullptr);
+        }
         if (hipEventSynchronize(e1) != hipSuccess) {
+            bf_batch_destroy(b);
             rc = kHipError;
             break;
         }
+        bf_batch_destroy(b);
         float ms = 0.0f;
         hipEventElapsedTime(&ms, e0, e1);
         if (r2 == kOk) cost[c] = ms / reps / ((float)width * height / 1e6f);
@@ -799,6 +808,12 @@ int bf_batch_create(const bf_job *jobs, int32_t njobs, int32_t config, bf_batch 
     }
     *out = reinterpret_cast<bf_batch>(B);
     return kOk;
+}
+
+int bf_batch_config(bf_batch h)
+{
+    if (!h) return kNullPointer;
+    return reinterpret_cast<Batch *>(h)->config;
 }
 
 int bf_batch_unrolled(bf_batch h)

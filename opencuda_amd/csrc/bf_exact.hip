@@ -44,64 +44,27 @@ namespace bfimpl {
 
 namespace {
 
-template <int TW, int TGY, int HALO, bool SEP>
+// A lane owns CPL adjacent columns (CPL * lx ..) of one 4-row group; its
+// columns' fetches of a tap are adjacent words of the staged tile (one
+// ds_read2_b32 for CPL = 2).  Every pixel runs its own taps in tap order,
+// so CPL changes no result.
+template <int TW, int TGY, int HALO, int CPL, bool SEP>
 __device__ __forceinline__ void tile_ref(const DevJob &job, int X0, int Y0, int lx, int gy)
 {
     using G = Geo<TW, TGY, HALO>;
     constexpr uint32_t LUT_BASE = kTileOff + G::TILE_BYTES;
     constexpr int ROWB = G::TCOLS * 4;
-    const uint32_t lane = kTileOff + (uint32_t)(((4 * gy + HALO) * G::TCOLS + lx + HALO) * 4);
-    const uint32_t c0 = lds_u32(lane), c1 = lds_u32(lane + ROWB);
-    const uint32_t c2 = lds_u32(lane + 2 * ROWB), c3 = lds_u32(lane + 3 * ROWB);
-    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
-    const BF_CONST uint32_t *taps = as_const(job.taps);
-    const BF_CONST float *gtap = as_const(job.gtap);
-    const int nt = job.ntaps_padded;
-    for (int c = 0; c < nt; c += kTapPad) {
+    const uint32_t lane = kTileOff + (uint32_t)(((4 * gy + HALO) * G::TCOLS + CPL * lx + HALO) * 4);
+    uint32_t cen[4][CPL];
+    float t[4][CPL], sum[4][CPL];
 #pragma unroll
-        for (int u = 0; u < kTapPad; u++) {
-            const uint32_t info = taps[c + u];
-            const int dy = (int)(int8_t)(info & 0xFFu);
-            const int dx = (int)(int8_t)((info >> 8) & 0xFFu);
-            const uint32_t wrow = SEP ? LUT_BASE : LUT_BASE + ((info >> 16) << 10);
-            const float g = SEP ? gtap[c + u] : 1.0f;
-            const uint32_t p = lds_u32(lane + (uint32_t)((dy * G::TCOLS + dx) * 4));
-            const float pf = (float)(p & 0xFFu);
-            float f0 = lds_f32(__builtin_amdgcn_sad_u8(p, c0, wrow));
-            float f1 = lds_f32(__builtin_amdgcn_sad_u8(p, c1, wrow));
-            float f2 = lds_f32(__builtin_amdgcn_sad_u8(p, c2, wrow));
-            float f3 = lds_f32(__builtin_amdgcn_sad_u8(p, c3, wrow));
-            if (SEP) {   // factor = g[col] * E[d] (BilateralFilter.cu:125), one rounding
-                f0 = g * f0; f1 = g * f1; f2 = g * f2; f3 = g * f3;
-            }
-            t0 = t0 + f0 * pf; s0 = s0 + f0;
-            t1 = t1 + f1 * pf; s1 = s1 + f1;
-            t2 = t2 + f2 * pf; s2 = s2 + f2;
-            t3 = t3 + f3 * pf; s3 = s3 + f3;
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int j = 0; j < CPL; j++) {
+            cen[k][j] = lds_u32(lane + k * ROWB + 4 * j);
+            t[k][j] = 0.0f;
+            sum[k][j] = 0.0f;
         }
-    }
-    const int x = X0 + lx, y0 = Y0 + 4 * gy;
-    if (x < job.W) {
-        uint8_t *o = job.out + (int64_t)y0 * job.out_pitch + x;
-        if (y0 < job.H) o[0] = to_u8(t0 / s0);
-        if (y0 + 1 < job.H) o[job.out_pitch] = to_u8(t1 / s1);
-        if (y0 + 2 < job.H) o[2 * job.out_pitch] = to_u8(t2 / s2);
-        if (y0 + 3 < job.H) o[3 * job.out_pitch] = to_u8(t3 / s3);
-    }
-}
-
-template <int TW, int TGY, int HALO, bool SEP>
-__device__ __forceinline__ void tile_std(const DevJob &job, int X0, int Y0, int lx, int gy)
-{
-    using G = Geo<TW, TGY, HALO>;
-    constexpr uint32_t LUT_BASE = kTileOff + G::TILE_BYTES;
-    constexpr int ROWB = G::TCOLS * 4;
-    const uint32_t lane = kTileOff + (uint32_t)(((4 * gy + HALO) * G::TCOLS + lx + HALO) * 4);
-    const uint32_t c0 = lds_u32(lane), c1 = lds_u32(lane + ROWB);
-    const uint32_t c2 = lds_u32(lane + 2 * ROWB), c3 = lds_u32(lane + 3 * ROWB);
-    float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
-    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
     const BF_CONST uint32_t *taps = as_const(job.taps);
     const BF_CONST float *gtap = as_const(job.gtap);
     const int nt = job.ntaps_padded;
@@ -114,38 +77,96 @@ __device__ __forceinline__ void tile_std(const DevJob &job, int X0, int Y0, int 
             const uint32_t wrow = SEP ? LUT_BASE : LUT_BASE + ((info >> 16) << 10);
             const float g = SEP ? gtap[c + u] : 1.0f;
             const uint32_t a = lane + (uint32_t)((dy * G::TCOLS + dx) * 4);
-            const uint32_t p0 = lds_u32(a), p1 = lds_u32(a + ROWB);
-            const uint32_t p2 = lds_u32(a + 2 * ROWB), p3 = lds_u32(a + 3 * ROWB);
-            float f0 = lds_f32(__builtin_amdgcn_sad_u8(p0, c0, wrow));
-            float f1 = lds_f32(__builtin_amdgcn_sad_u8(p1, c1, wrow));
-            float f2 = lds_f32(__builtin_amdgcn_sad_u8(p2, c2, wrow));
-            float f3 = lds_f32(__builtin_amdgcn_sad_u8(p3, c3, wrow));
-            if (SEP) {
-                f0 = g * f0; f1 = g * f1; f2 = g * f2; f3 = g * f3;
+#pragma unroll
+            for (int j = 0; j < CPL; j++) {
+                const uint32_t p = lds_u32(a + 4 * j);
+                const float pf = (float)(p & 0xFFu);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    float f = lds_f32(__builtin_amdgcn_sad_u8(p, cen[k][j], wrow));
+                    if (SEP) f = g * f;   // factor = g[col] * E[d] (BilateralFilter.cu:125), one rounding
+                    t[k][j] = t[k][j] + f * pf;
+                    sum[k][j] = sum[k][j] + f;
+                }
             }
-            t0 = t0 + f0 * (float)(p0 & 0xFFu); s0 = s0 + f0;
-            t1 = t1 + f1 * (float)(p1 & 0xFFu); s1 = s1 + f1;
-            t2 = t2 + f2 * (float)(p2 & 0xFFu); s2 = s2 + f2;
-            t3 = t3 + f3 * (float)(p3 & 0xFFu); s3 = s3 + f3;
         }
     }
-    const int x = X0 + lx, y0 = Y0 + 4 * gy;
-    if (x < job.W) {
-        uint8_t *o = job.out + (int64_t)y0 * job.out_pitch + x;
-        if (y0 < job.H) o[0] = to_u8(t0 / s0);
-        if (y0 + 1 < job.H) o[job.out_pitch] = to_u8(t1 / s1);
-        if (y0 + 2 < job.H) o[2 * job.out_pitch] = to_u8(t2 / s2);
-        if (y0 + 3 < job.H) o[3 * job.out_pitch] = to_u8(t3 / s3);
+    const int y0 = Y0 + 4 * gy;
+#pragma unroll
+    for (int j = 0; j < CPL; j++) {
+        const int x = X0 + CPL * lx + j;
+        if (x < job.W) {
+            uint8_t *o = job.out + (int64_t)y0 * job.out_pitch + x;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (y0 + k < job.H) o[k * job.out_pitch] = to_u8(t[k][j] / sum[k][j]);
+        }
     }
 }
 
-template <int TW, int TGY, int HALO>
-__global__ void __launch_bounds__(TW * TGY)
+template <int TW, int TGY, int HALO, int CPL, bool SEP>
+__device__ __forceinline__ void tile_std(const DevJob &job, int X0, int Y0, int lx, int gy)
+{
+    using G = Geo<TW, TGY, HALO>;
+    constexpr uint32_t LUT_BASE = kTileOff + G::TILE_BYTES;
+    constexpr int ROWB = G::TCOLS * 4;
+    const uint32_t lane = kTileOff + (uint32_t)(((4 * gy + HALO) * G::TCOLS + CPL * lx + HALO) * 4);
+    uint32_t cen[4][CPL];
+    float t[4][CPL], sum[4][CPL];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int j = 0; j < CPL; j++) {
+            cen[k][j] = lds_u32(lane + k * ROWB + 4 * j);
+            t[k][j] = 0.0f;
+            sum[k][j] = 0.0f;
+        }
+    const BF_CONST uint32_t *taps = as_const(job.taps);
+    const BF_CONST float *gtap = as_const(job.gtap);
+    const int nt = job.ntaps_padded;
+    for (int c = 0; c < nt; c += kTapPad) {
+#pragma unroll
+        for (int u = 0; u < kTapPad; u++) {
+            const uint32_t info = taps[c + u];
+            const int dy = (int)(int8_t)(info & 0xFFu);
+            const int dx = (int)(int8_t)((info >> 8) & 0xFFu);
+            const uint32_t wrow = SEP ? LUT_BASE : LUT_BASE + ((info >> 16) << 10);
+            const float g = SEP ? gtap[c + u] : 1.0f;
+            const uint32_t a = lane + (uint32_t)((dy * G::TCOLS + dx) * 4);
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+#pragma unroll
+                for (int j = 0; j < CPL; j++) {
+                    const uint32_t p = lds_u32(a + k * ROWB + 4 * j);
+                    float f = lds_f32(__builtin_amdgcn_sad_u8(p, cen[k][j], wrow));
+                    if (SEP) f = g * f;
+                    t[k][j] = t[k][j] + f * (float)(p & 0xFFu);
+                    sum[k][j] = sum[k][j] + f;
+                }
+        }
+    }
+    const int y0 = Y0 + 4 * gy;
+#pragma unroll
+    for (int j = 0; j < CPL; j++) {
+        const int x = X0 + CPL * lx + j;
+        if (x < job.W) {
+            uint8_t *o = job.out + (int64_t)y0 * job.out_pitch + x;
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (y0 + k < job.H) o[k * job.out_pitch] = to_u8(t[k][j] / sum[k][j]);
+        }
+    }
+}
+
+template <int TW, int TGY, int HALO, int CPL>
+__global__ void __launch_bounds__(TW / CPL * TGY)
 bf_exact_kernel(KArgs args)
 {
     using G = Geo<TW, TGY, HALO>;
+    constexpr int LX = TW / CPL;              // lanes across the tile
+    constexpr int NT = LX * TGY;
     const int tid = threadIdx.x;
-    const int lx = tid % TW, gy = tid / TW;
+    const int lx = tid % LX, gy = tid / LX;
     const int ntiles = args.tile_list ? *as_const(args.tile_count) : args.ntiles;
     const BF_CONST DevJob *jobs = as_const(args.jobs);
     int cur_key = -1;
@@ -179,21 +200,21 @@ bf_exact_kernel(KArgs args)
         const int Y0 = ty * G::TH;
         __syncthreads();                           // previous tile fully consumed
         if (job.lut_key != cur_key) {
-            build_lut<G::NT>(job, kTileOff + (uint32_t)G::TILE_BYTES, tid);
+            build_lut<NT>(job, kTileOff + (uint32_t)G::TILE_BYTES, tid);
             cur_key = job.lut_key;
         }
-        load_tile<TW, TGY, HALO>(job, X0, Y0, tid);
+        stage_rect<G::TCOLS, G::TROWS, NT, false>(job, X0 - HALO, Y0 - HALO, kTileOff, tid);
         __syncthreads();
         if (job.sep) {
             if (job.mode == kModeRef)
-                tile_ref<TW, TGY, HALO, true>(job, X0, Y0, lx, gy);
+                tile_ref<TW, TGY, HALO, CPL, true>(job, X0, Y0, lx, gy);
             else
-                tile_std<TW, TGY, HALO, true>(job, X0, Y0, lx, gy);
+                tile_std<TW, TGY, HALO, CPL, true>(job, X0, Y0, lx, gy);
         } else {
             if (job.mode == kModeRef)
-                tile_ref<TW, TGY, HALO, false>(job, X0, Y0, lx, gy);
+                tile_ref<TW, TGY, HALO, CPL, false>(job, X0, Y0, lx, gy);
             else
-                tile_std<TW, TGY, HALO, false>(job, X0, Y0, lx, gy);
+                tile_std<TW, TGY, HALO, CPL, false>(job, X0, Y0, lx, gy);
         }
     }
 }
@@ -374,25 +395,30 @@ __global__ void reset_counter_kernel(int32_t *counter) { *counter = 0; }
 
 // ---------------------------------------------------------------- configs
 
+// tile width, 4-row groups, threads, columns per lane
 constexpr TileConfig kConfigs[] = {
-    { 64, 16, 1024 },   // 0: 64 x 64 tile
-    { 64, 8, 512 },     // 1: 64 x 32
-    { 32, 8, 256 },     // 2: 32 x 32 (the reference's block footprint)
-    { 64, 4, 256 },     // 3: 64 x 16
-    { 128, 4, 512 },    // 4: 128 x 16
-    { 128, 8, 1024 },   // 5: 128 x 32
+    { 64, 16, 1024, 1 },   // 0: 64 x 64 tile
+    { 64, 8, 512, 1 },     // 1: 64 x 32
+    { 32, 8, 256, 1 },     // 2: 32 x 32 (the reference's block footprint)
+    { 64, 4, 256, 1 },     // 3: 64 x 16
+    { 128, 4, 512, 1 },    // 4: 128 x 16
+    { 128, 8, 1024, 1 },   // 5: 128 x 32
+    { 64, 16, 512, 2 },    // 6: 64 x 64, two columns per lane
+    { 128, 8, 512, 2 },    // 7: 128 x 32, two columns per lane
+    { 128, 4, 256, 2 },    // 8: 128 x 16, two columns per lane
 };
 constexpr int kNumConfigs = sizeof(kConfigs) / sizeof(kConfigs[0]);
 constexpr int kHalos[] = { 4, 8, 12, 16, 20 };
 
-template <int TW, int TGY, int HALO>
+template <int TW, int TGY, int HALO, int CPL>
 int launch_one(const KArgs &a, int max_lut_rows, hipStream_t s)
 {
     using G = Geo<TW, TGY, HALO>;
+    constexpr int NT = TW / CPL * TGY;
     const size_t lds = kTileOff + (size_t)G::TILE_BYTES + (size_t)max_lut_rows * 1024;
-    constexpr auto kern = bf_exact_kernel<TW, TGY, HALO>;
+    constexpr auto kern = bf_exact_kernel<TW, TGY, HALO, CPL>;
     BF_HIP(allow_full_lds<kern>());
-    const int res = resident_groups(reinterpret_cast<const void *>(kern), G::NT, lds);
+    const int res = resident_groups(reinterpret_cast<const void *>(kern), NT, lds);
     if (res < 0) return hip_fail(hipGetLastError(), "occupancy query");
     if (res < 1) {
         set_error("exact kernel does not fit on a CU (LDS " + std::to_string(lds) + " bytes)");
@@ -402,20 +428,20 @@ int launch_one(const KArgs &a, int max_lut_rows, hipStream_t s)
     if (!a.tile_list && !a.counter && a.ntiles < grid) grid = a.ntiles;
     if (a.tile_list && a.ntiles < grid) grid = a.ntiles;   // ntiles = max possible
     if (grid < 1) return kOk;
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(G::NT), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
     BF_HIP(hipGetLastError());
     return kOk;
 }
 
-template <int TW, int TGY>
+template <int TW, int TGY, int CPL = 1>
 int launch_halo(int halo, const KArgs &a, int max_lut_rows, hipStream_t s)
 {
     switch (halo) {
-    case 4: return launch_one<TW, TGY, 4>(a, max_lut_rows, s);
-    case 8: return launch_one<TW, TGY, 8>(a, max_lut_rows, s);
-    case 12: return launch_one<TW, TGY, 12>(a, max_lut_rows, s);
-    case 16: return launch_one<TW, TGY, 16>(a, max_lut_rows, s);
-    case 20: return launch_one<TW, TGY, 20>(a, max_lut_rows, s);
+    case 4: return launch_one<TW, TGY, 4, CPL>(a, max_lut_rows, s);
+    case 8: return launch_one<TW, TGY, 8, CPL>(a, max_lut_rows, s);
+    case 12: return launch_one<TW, TGY, 12, CPL>(a, max_lut_rows, s);
+    case 16: return launch_one<TW, TGY, 16, CPL>(a, max_lut_rows, s);
+    case 20: return launch_one<TW, TGY, 20, CPL>(a, max_lut_rows, s);
     default: set_error("bad halo class"); return kInvalidData;
     }
 }
@@ -468,6 +494,9 @@ int launch_exact(int config, int halo, const DevJob &job0, const DevJob *jobs_de
     case 3: return launch_halo<64, 4>(halo, a, max_lut_rows, s);
     case 4: return launch_halo<128, 4>(halo, a, max_lut_rows, s);
     case 5: return launch_halo<128, 8>(halo, a, max_lut_rows, s);
+    case 6: return launch_halo<64, 16, 2>(halo, a, max_lut_rows, s);
+    case 7: return launch_halo<128, 8, 2>(halo, a, max_lut_rows, s);
+    case 8: return launch_halo<128, 4, 2>(halo, a, max_lut_rows, s);
     default: set_error("bad tile config"); return kInvalidData;
     }
 }

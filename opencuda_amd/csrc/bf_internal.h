@@ -86,9 +86,10 @@ struct alignas(16) DevJob {
 };
 
 struct TileConfig {
-    int tw;        // tile width = lanes along x
+    int tw;        // tile width in pixels
     int tgy;       // 4-row groups along y; tile height = 4*tgy
-    int threads;   // tw * tgy
+    int threads;   // tw / cpl * tgy
+    int cpl;       // adjacent columns per lane
 };
 
 int tile_config_count();

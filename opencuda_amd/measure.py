@@ -121,9 +121,20 @@ def c2(steps: int = 5, mode: int = MODE_REF, tune_table: str | None = None):
         bt.run_graph(stream)
         torch.cuda.synchronize()
         ms_tuned = _time(lambda: bt.run_graph(stream), steps, stream)
-        tuned = {"entries": n, "graph_ms": round(ms_tuned, 4),
-                 "graph_mpix_s": round(px / ms_tuned / 1e3, 1)}
-        _log(f"with the tuned table ({n} entries): graph {ms_tuned:.3f} ms")
+        best, times = autotune.tune_batch(jobs, reps=steps, stream=stream)
+        bb = Batch(jobs, best)
+        bb.run_graph(stream)
+        torch.cuda.synchronize()
+        ms_best = _time(lambda: bb.run_graph(stream), steps, stream)
+        tuned = {"entries": n, "table_config": bt.config, "graph_ms": round(ms_tuned, 4),
+                 "graph_mpix_s": round(px / ms_tuned / 1e3, 1),
+                 "batch_tuned": {"config": best,
+                                 "ms_per_config": [None if t is None else round(t, 4)
+                                                   for t in times],
+                                 "graph_ms": round(ms_best, 4),
+                                 "graph_mpix_s": round(px / ms_best / 1e3, 1)}}
+        _log(f"with the tuned table ({n} entries, config {bt.config}): graph {ms_tuned:.3f} ms; "
+             f"batch tuned on itself: config {best}, graph {ms_best:.3f} ms")
     by_r = {}
     for it in items:
         by_r[it.radius] = by_r.get(it.radius, 0) + 1

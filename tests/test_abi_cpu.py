@@ -94,8 +94,11 @@ def test_argument_validation_without_gpu():
 def test_tile_configs_describe():
     cfgs = oc.tune_configs()
     assert len(cfgs) >= 4
-    for c, tw, th, nt in cfgs:
-        assert tw * th // 4 == nt and th % 4 == 0 and nt % 64 == 0
+    for c, tw, th, nt, cpl in cfgs:
+        assert tw * th // 4 // cpl == nt and th % 4 == 0 and nt % 64 == 0 and tw % cpl == 0
+    # pixels per lane is a tuned dimension: some configurations give a lane
+    # two columns (8 pixels), the others one (4 pixels)
+    assert {c[4] for c in cfgs} == {1, 2}
     assert oc.lib().bf_tune_describe(999, None, None, None, None) == oc.INVALID_DATA
 
 

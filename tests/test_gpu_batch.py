@@ -75,7 +75,7 @@ def test_batch_mixed_modes_and_table_modes(device):
 
 def test_every_config_identical(device):
     items = wl.c2(10)
-    for cfg, tw, th, nt in oc.tune_configs():
+    for cfg, tw, th, nt, cpl in oc.tune_configs():
         bc = BatchCase(items, config=cfg)
         assert bc.batch.unrolled == (cfg in (0, 1))
         bc.batch.run()

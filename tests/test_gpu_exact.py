@@ -68,7 +68,7 @@ def test_every_tile_config_identical(device, radius, ss, sr):
     img = _img(150, 210, seed=radius)
     roi = (7, 5, 205, 148)
     ref = None
-    for cfg, tw, th, nt in oc.tune_configs():
+    for cfg, tw, th, nt, cpl in oc.tune_configs():
         for mode in (oc.MODE_REF, oc.MODE_STD):
             try:
                 got = run_exact(img, radius, ss, sr, mode, roi=roi, config=cfg)
