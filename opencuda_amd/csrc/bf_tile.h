@@ -196,6 +196,8 @@ __device__ __forceinline__ void build_lut(const DevJob &job, uint32_t lut_base, 
 
 struct RingTap {
     int dy, dx, row;
+    int side;      // 0: the ring's top/bottom rows; 1: right column; 2: left column
+    bool first;    // first tap of its side column in this ring (j = -i + 1)
 };
 
 template <int R, int MODE>
@@ -219,7 +221,8 @@ struct RingTaps {
                         qs[nrows] = q;
                         row = nrows++;
                     }
-                    t[n++] = RingTap{ dy, dx, row };
+                    const int side = k < 2 ? 0 : k - 1;
+                    t[n++] = RingTap{ dy, dx, row, side, side != 0 && j == -i + 1 };
                 }
             }
         }
@@ -276,6 +279,8 @@ __device__ __forceinline__ void tile_ring(const DevJob &job, int X0, int Y0, int
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
     constexpr int MINB = ring_min_offset<R, MODE, G::TCOLS>() * 4;
     const uint32_t tbase = lane + (uint32_t)MINB;   // >= 0: the halo covers every tap
+    uint32_t wp[2][4] = {};                         // STD: side-column windows (right, left)
+    float wf[2][4] = {};
     // Gathers address W[row][d] as (4d + LUT_BASE) + row*1024; rows past 63
     // go through a second base so every offset fits the 16-bit field.
     static_for<T>([&](auto ci) {
@@ -295,16 +300,47 @@ __device__ __forceinline__ void tile_ring(const DevJob &job, int X0, int Y0, int
             t2 = t2 + f2 * pf; s2 = s2 + f2;
             t3 = t3 + f3 * pf; s3 = s3 + f3;
         } else {
-            const uint32_t p0 = lds_u32(tbase + TAPB), p1 = lds_u32(tbase + TAPB + ROWB);
-            const uint32_t p2 = lds_u32(tbase + TAPB + 2 * ROWB), p3 = lds_u32(tbase + TAPB + 3 * ROWB);
+            // Own window per row: row k reads tile row k + dy.  Along a side
+            // column of a ring (dx = +-i, dy = j = -i+1 .. i, taps four apart
+            // in ring order) row k at dy = j reads the pixel row k+1 read at
+            // dy = j - 1: a window of the four pixels rows j .. j+3 of that
+            // column (bytes and their float values) slides down one row per
+            // tap, so such a tap fetches and converts one pixel, not four.
+            uint32_t p0, p1, p2, p3;
+            float q0, q1, q2, q3;
+            if constexpr (tp.side != 0) {
+                constexpr int sd = tp.side - 1;
+                if constexpr (tp.first) {
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        wp[sd][m] = lds_u32(tbase + TAPB + m * ROWB);
+                        wf[sd][m] = (float)(wp[sd][m] & 0xFFu);
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 3; m++) {
+                        wp[sd][m] = wp[sd][m + 1];
+                        wf[sd][m] = wf[sd][m + 1];
+                    }
+                    wp[sd][3] = lds_u32(tbase + TAPB + 3 * ROWB);
+                    wf[sd][3] = (float)(wp[sd][3] & 0xFFu);
+                }
+                p0 = wp[sd][0]; p1 = wp[sd][1]; p2 = wp[sd][2]; p3 = wp[sd][3];
+                q0 = wf[sd][0]; q1 = wf[sd][1]; q2 = wf[sd][2]; q3 = wf[sd][3];
+            } else {
+                p0 = lds_u32(tbase + TAPB); p1 = lds_u32(tbase + TAPB + ROWB);
+                p2 = lds_u32(tbase + TAPB + 2 * ROWB); p3 = lds_u32(tbase + TAPB + 3 * ROWB);
+                q0 = (float)(p0 & 0xFFu); q1 = (float)(p1 & 0xFFu);
+                q2 = (float)(p2 & 0xFFu); q3 = (float)(p3 & 0xFFu);
+            }
             const float f0 = lds_f32(__builtin_amdgcn_sad_u8(p0, c0, WB) + WOFF);
             const float f1 = lds_f32(__builtin_amdgcn_sad_u8(p1, c1, WB) + WOFF);
             const float f2 = lds_f32(__builtin_amdgcn_sad_u8(p2, c2, WB) + WOFF);
             const float f3 = lds_f32(__builtin_amdgcn_sad_u8(p3, c3, WB) + WOFF);
-            t0 = t0 + f0 * (float)(p0 & 0xFFu); s0 = s0 + f0;
-            t1 = t1 + f1 * (float)(p1 & 0xFFu); s1 = s1 + f1;
-            t2 = t2 + f2 * (float)(p2 & 0xFFu); s2 = s2 + f2;
-            t3 = t3 + f3 * (float)(p3 & 0xFFu); s3 = s3 + f3;
+            t0 = t0 + f0 * q0; s0 = s0 + f0;
+            t1 = t1 + f1 * q1; s1 = s1 + f1;
+            t2 = t2 + f2 * q2; s2 = s2 + f2;
+            t3 = t3 + f3 * q3; s3 = s3 + f3;
         }
         // Keep the schedule local (a group of taps in flight), as the loop
         // version does: without a fence the compiler issues every gather of
