@@ -39,6 +39,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# The headline metric's name as the benchmark definition states it.  Its
+# "(≤2 LSB)" is the surrogate check's tolerance on sampled pixels; the
+# surrogate block reports whether every pixel met it (it does not).
 METRIC = "Mpix/s r=15 on 8192² u8: exact FP32 vs BF16 surrogate+fallback (≤2 LSB)"
 PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector nameplate (packed FMA)
 PEAK_FP32_NOFMA_TFLOPS = 78.6  # packed mul/add, no FMA
@@ -207,6 +210,13 @@ def surrogate_leg(args, it, img, tin, lut, std_out, std_ms, ref_ms, stream):
                            "err_max": int(dp.max()), "err_mean": round(float(dp.mean()), 4),
                            "within_2lsb": round(float((dp <= 2).mean()), 6)},
         "target": "exact STD mode (own window, 961 taps)",
+        # The tolerance is that of a sampled check (one hashed pixel per 8x8
+        # cell of each 64x64 tile), not a per-pixel bound: pixels the samples
+        # miss may stay further off, as err_vs_exact_std records.
+        "check": "1/64 sampled exact check per 64x64 tile, tile refiltered if any sampled "
+                 "|diff| > 2 LSB",
+        "every_pixel_within_2lsb": bool(d.max() <= 2),
+        "pixels_beyond_2lsb": int((d > 2).sum()),
         "weights": os.path.relpath(sg.DEFAULT_WEIGHTS, ROOT),
         "trained_steps": int(meta.get("steps", 0)),
     }
