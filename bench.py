@@ -84,8 +84,11 @@ def cpu_baseline(img, it, gpu_out, seconds: float):
                             rows=(y0, y0 + rows), nthreads=nt)
         return out, time.perf_counter() - t
 
-    _, dt = run(8, nthreads)                          # size the band
-    rows = int(max(8, min(H - y0, seconds / 4 / max(dt, 1e-6) * 8)) // 4 * 4)
+    # size the band: the first call pays thread start-up and cold caches, so
+    # the rate comes from a second, larger call
+    run(8, nthreads)
+    _, dt = run(64, nthreads)
+    rows = int(max(8, min(H - y0, seconds / 4 / max(dt, 1e-6) * 64)) // 4 * 4)
     log(f"cpu baseline: {rows} rows x {img.shape[1]} px on {nthreads} threads "
         f"({'native' if build['native'] else 'x86-64-v3'} build), warm-up + 3 runs")
     band, _ = run(rows, nthreads)                     # warm-up
