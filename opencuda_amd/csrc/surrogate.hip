@@ -92,7 +92,9 @@ __device__ __forceinline__ uint32_t act_pack2(float lo, float hi)
 {
     typedef short i16x2_t __attribute__((ext_vector_type(2)));
     typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
-    const bf16x2_t b = { to_bf(lo), to_bf(hi) };
+    typedef float f32x2_t __attribute__((ext_vector_type(2)));
+    // one v_cvt_pk_bf16_f32 for both (round to nearest even, as to_bf)
+    const bf16x2_t b = __builtin_convertvector((f32x2_t){ lo, hi }, bf16x2_t);
     const i16x2_t v = __builtin_elementwise_max(__builtin_bit_cast(i16x2_t, b), (i16x2_t){ 0, 0 });
     return __builtin_bit_cast(uint32_t, v);
 }
@@ -810,25 +812,31 @@ __global__ void pack_kernel(const float *params, PackTables pk)
 // All six layers of the forward pass in one kernel, for inference: no
 // activation leaves the chip.  A workgroup owns a strip of FS_W output
 // columns and a band of rows and streams down the band two rows per step.
-// At step k, layer l (0..4) computes rows Y0 + 2k + 3(5-l) and the row
-// below (64 columns starting 5-l left of the strip: the context the layers
-// above need); layer l+1 reads them at steps k+1 and k+2 (as its rows
-// r-1 .. r+2), so each layer keeps a ring of FS_RING = 6 rows in LDS.
+// At step k, hidden layer l (1..4) computes rows Y0 + 2k + 20 - 4l and the
+// row below, the first layer (0) rows Y0 + 2k + 19 and + 20, and the output
+// rows are Y0 + 2k and + 1 (64 columns starting 5-l left of the strip for
+// layer l: the context the layers above need).  A hidden layer writes its
+// first row at the end of the step and its second row during the next step
+// (its epilogue overlaps that step's first MFMAs), so the layer above, four
+// rows behind, reads a row two steps after it was computed; each activation
+// ring holds FS_RING = 4 rows, the last layer's product ring FS_DRING = 8.
 //
 // Eight waves, two per SIMD.  Wave w runs hidden layer 1 + w/2 on M-tiles
 // 2(w%2) and 2(w%2)+1 (16 pixels each), both rows, both 16-channel N-tiles:
 // 72 MFMAs per step.  The fragments of a layer's input rows r+1 and r+2 are
 // the input rows r'-1 and r' of the next step, so they stay in registers:
 // each step reads only two new input rows from LDS, and starts on the
-// fragments it already holds while the new ones arrive.  Waves 0..3 also
+// fragments it already holds (and on the previous step's deferred
+// epilogue) while the new ones arrive.  Waves 0..3 also
 // run M-tile w of the first layer (its input is a ring of image dwords
 // bf16(u/255) | bf16(sigma_r/100) << 16, zero outside the image); waves 6,7
 // (layer 4) feed their own BF16 outputs straight from registers into the
 // last layer, computed per pixel as D[t][x] = sum_ci W6[t][ci] * act[x][ci]
 // for the nine taps t (one MFMA with the taps as rows), written to a ring of
 // FP32 products; waves 4,5 sum each output pixel's nine products in tap
-// order, add the residual and store, and prefetch image rows 16 at a time
-// into registers eight steps before the ring needs them.
+// order, add the residual and store, and stream the image: each step they
+// write two image rows loaded into registers the step before and load the
+// next two, so a load has a whole step to arrive.
 //
 // The MFMAs compute the transposed product (weights as the A operand,
 // activations as B), so a lane ends with four consecutive channels of one
@@ -844,7 +852,8 @@ __global__ void pack_kernel(const float *params, PackTables pk)
 
 constexpr int FS_W = 54;                  // output columns per strip
 constexpr int FS_WAVES = 8;
-constexpr int FS_RING = 6;                // rows per activation / product ring
+constexpr int FS_RING = 4;                // rows per activation ring
+constexpr int FS_DRING = 8;               // rows of the last layer's product ring
 constexpr int FS_ROWB = 64 * 64;          // bytes per activation ring row
 constexpr int FS_D_TAPS = 16;             // product rows per ring row (9 taps used)
 constexpr int FS_D_PITCH = 68;            // words per product row (64 columns + pad)
@@ -854,19 +863,18 @@ constexpr int FS_IMG_ROWS = 64;           // image ring rows
 constexpr int FS_IMG_DUP = 3;             // ring rows 0..2 repeated after row 63
 constexpr int FS_IMG_PITCH = 80;          // image ring columns
 constexpr int FS_IMG_W = 66;              // image columns a strip reads
-constexpr int FS_IMGW_OFF = FS_D_OFF + FS_RING * FS_D_ROWB;            // dword ring (first layer)
+constexpr int FS_IMGW_OFF = FS_D_OFF + FS_DRING * FS_D_ROWB;           // dword ring (first layer)
 constexpr int FS_IMGU_OFF = FS_IMGW_OFF + (FS_IMG_ROWS + FS_IMG_DUP) * FS_IMG_PITCH * 4;  // u8 ring
-constexpr int FS_LUT_OFF = FS_IMGU_OFF + FS_IMG_ROWS * FS_IMG_PITCH;   // bf16(u / 255), u = 0..255
-constexpr int FS_CB_OFF = FS_LUT_OFF + 512;                           // biases of layers 0..4 (32 floats each)
+constexpr int FS_LUT_OFF = FS_IMGU_OFF + FS_IMG_ROWS * FS_IMG_PITCH;   // image dword of u = 0..255
+constexpr int FS_CB_OFF = FS_LUT_OFF + 256 * 4;                       // biases of layers 0..4 (32 floats each)
 constexpr int FS_W0_OFF = FS_CB_OFF + 5 * 32 * 4;                      // first layer's A operand (64 lanes x 2 x 16 B)
 constexpr int FS_W6_OFF = FS_W0_OFF + 2 * 64 * 16;                     // last layer's A operand (64 lanes x 16 B)
-constexpr int FS_LDS = FS_W6_OFF + 64 * 16;
-constexpr int FS_CHUNK = 16;              // image rows per prefetch
-constexpr int FS_PRE = 20;                // image rows loaded before the first step
-constexpr int FS_PF = 9;                  // prefetched bytes per thread of waves 4, 5
-constexpr int FS_KBEGIN = -10;            // first step: layer 0 computes rows Y0-5, Y0-4
-constexpr int FS_CHUNK_BYTES = FS_CHUNK * FS_IMG_W;
-static_assert(FS_CHUNK_BYTES <= FS_PF * 128, "a prefetch fits the registers of waves 4, 5");
+constexpr int FS_RES_OFF = FS_W6_OFF + 64 * 16;                        // residual u / 255 (FP32), u = 0..255
+constexpr int FS_LDS = FS_RES_OFF + 256 * 4;
+constexpr int FS_PRE = 4;                 // image rows written before the first step
+constexpr int FS_ROW2 = 2 * FS_IMG_W;     // bytes of the two image rows a step streams
+constexpr int FS_KBEGIN = -12;            // first step: layer 0 computes rows Y0-5, Y0-4
+static_assert(FS_ROW2 <= 2 * 128, "two image rows fit two bytes per thread of waves 4, 5");
 static_assert(FS_LDS <= 160 * 1024, "LDS");
 static_assert(FS_IMG_PITCH >= FS_IMG_W, "image ring rows");
 
@@ -938,9 +946,9 @@ __device__ __forceinline__ uint8_t fs_load_img(const FusedArgs &a, int row, int 
     return a.img[(int64_t)row * a.img_pitch + col];
 }
 
-// Slot of a row in a 6-row ring (any row >= Y0 - 60; the steps touch rows
-// from Y0 - 21 on).
-__device__ __forceinline__ int fs_slot6(int row, int Y0) { return (row - Y0 + 60) % FS_RING; }
+// Slot of a row in an activation ring and in the product ring.
+__device__ __forceinline__ int fs_slot(int row, int Y0) { return (row - Y0) & (FS_RING - 1); }
+__device__ __forceinline__ int fs_dslot(int row, int Y0) { return (row - Y0) & (FS_DRING - 1); }
 
 template <int I> using fs_ic = std::integral_constant<int, I>;
 
@@ -957,7 +965,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const int H = a.H, W = a.W;
     const int img_base = Y0 - 6;              // image row of ring row 0
     const int img_x = x0 - 6;                 // image column of ring column 0
-    const int ptid = tid - 256;               // prefetch thread (waves 4, 5)
+    const int ptid = tid - 256;               // image streaming thread (waves 4, 5)
 
     auto img_slot = [&](int row) { return (row - img_base) & (FS_IMG_ROWS - 1); };
 
@@ -990,7 +998,6 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     }
     const uint32_t h_in = (uint32_t)((L - 1) * FS_RING * FS_ROWB);
     const uint32_t h_out = (uint32_t)(L * FS_RING * FS_ROWB);     // unused by layer 4
-    const int h_lo = Y0 - (5 - L), h_hi = Y1 + (5 - L);           // rows layer L must compute
 
     // ---- role data
     const bool first_w = wave < 4;            // first layer, M-tile `wave`
@@ -1013,7 +1020,12 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const float b6 = a.bias[NLAYERS - 1][0];
 
     // ---- tables, per-layer constants and the first FS_PRE image rows
-    if (tid < 256) fs_write_bf(FS_LUT_OFF + 2 * tid, to_bf((float)tid / 255.0f));
+    if (tid < 256) {
+        // the first layer's input dword of a pixel u inside the image
+        const uint32_t sr_hi = (uint32_t)__builtin_bit_cast(uint16_t, to_bf(a.sr)) << 16;
+        fs_write_u32(FS_LUT_OFF + 4 * tid, (uint32_t)__builtin_bit_cast(uint16_t, to_bf((float)tid / 255.0f)) | sr_hi);
+        fs_write_f32(FS_RES_OFF + 4 * tid, (float)tid / 255.0f);
+    }
     if (tid < 5 * 32) fs_write_f32(FS_CB_OFF + 4 * tid, a.bias[tid >> 5][tid & 31]);
     if (tid < 128) {
         // first layer A operand of lane (kq, i16), N-tile nt: W0[co = 16 nt + i16][k = 8 kq ..]
@@ -1028,13 +1040,12 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         if (t < 9) v = cat4(a.fw[NLAYERS - 1] + t * 32 + q4, a.fw[NLAYERS - 1] + t * 32 + 16 + q4);
         *(__attribute__((address_space(3))) bf16x8 *)(size_t)(FS_W6_OFF + 16 * l) = v;
     }
-    const uint32_t sr_hi = (uint32_t)__builtin_bit_cast(uint16_t, to_bf(a.sr)) << 16;
     fs_step_barrier();
     auto put_img = [&](int row, int cc, uint8_t u) {
         const int sl = img_slot(row);
         const int gx = img_x + cc;
         const bool in = row >= 0 && row < H && gx >= 0 && gx < W;
-        const uint32_t wv = in ? ((uint32_t)__builtin_bit_cast(uint16_t, fs_read_bf(FS_LUT_OFF + 2 * u)) | sr_hi) : 0u;
+        const uint32_t wv = in ? fs_read_u32(FS_LUT_OFF + 4 * u) : 0u;
         fs_write_u8(FS_IMGU_OFF + sl * FS_IMG_PITCH + cc, u);
         fs_write_u32(FS_IMGW_OFF + (sl * FS_IMG_PITCH + cc) * 4, wv);
         if (sl < FS_IMG_DUP) fs_write_u32(FS_IMGW_OFF + ((sl + FS_IMG_ROWS) * FS_IMG_PITCH + cc) * 4, wv);
@@ -1046,12 +1057,13 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     }
     fs_step_barrier();
 
-    // hidden layer bias: the accumulators' initial value
-    f32x4 bias[2];
+    // hidden layer bias, the accumulators' initial value: read from LDS
+    // before each use (two reads a step) rather than held in registers
+    const uint32_t bias_off = (uint32_t)(FS_CB_OFF + 128 * L + 16 * kq);
+    auto read_bias = [&](f32x4 (&b)[2]) __attribute__((always_inline)) {
 #pragma unroll
-    for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) bias[nt][r] = a.bias[L][16 * nt + 4 * kq + r];
+        for (int nt = 0; nt < 2; nt++) b[nt] = fs_read_f4(bias_off + 64 * nt);
+    };
 
     bf16x8 fr[3][3][2];                       // [row slot][dx][M-tile] fragments of layer L-1
 #pragma unroll
@@ -1060,9 +1072,32 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int mm = 0; mm < 2; mm++) fr[i][d][mm] = (bf16x8){};
-    uint32_t pf[(FS_PF + 3) / 4];             // prefetched image bytes, four per dword
+    // second-row accumulators, finished during the next step (the first
+    // step finishes these initial values into a row no needed row reads)
+    f32x4 acc1[2][2];
+    {
+        f32x4 b[2];
+        read_bias(b);
 #pragma unroll
-    for (int j = 0; j < (FS_PF + 3) / 4; j++) pf[j] = 0;
+        for (int mm = 0; mm < 2; mm++)
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) acc1[mm][nt] = b[nt];
+    }
+    // Image streaming (waves 4, 5): at step s = k - FS_KBEGIN the first layer
+    // reads image rows img_base + 2s .. + 2s + 3.  Step s writes rows
+    // img_base + 2s + 4 and + 5 into the ring (visible from step s + 1) from
+    // the bytes loaded at step s - 1, then loads rows + 6 and + 7.  Thread
+    // ptid streams bytes ptid and ptid + 128 of the two rows (the second
+    // only for ptid < FS_ROW2 - 128; the others load a valid byte and drop
+    // it).
+    const int pe0 = ptid, pe1 = min(ptid + 128, FS_ROW2 - 1);
+    const int prr0 = pe0 / FS_IMG_W, pcc0 = pe0 - prr0 * FS_IMG_W;
+    const int prr1 = pe1 / FS_IMG_W, pcc1 = pe1 - prr1 * FS_IMG_W;
+    uint8_t pb0 = 0, pb1 = 0;
+    if (tid >= 256 && tid < 384) {
+        pb0 = fs_load_img(a, img_base + FS_PRE + prr0, img_x + pcc0);
+        pb1 = fs_load_img(a, img_base + FS_PRE + prr1, img_x + pcc1);
+    }
 
     // Every layer computes its two rows at every step, needed or not: a row
     // outside a layer's needed range may hold garbage, but a needed row reads
@@ -1080,45 +1115,15 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         constexpr int RL = decltype(ROLE)::value;
         constexpr bool MK = decltype(MASKED)::value;
         constexpr int A = decltype(SA)::value, B = decltype(SB)::value, Cs = decltype(SC)::value;
-        const int r = Y0 + 2 * k + 3 * (5 - L);        // this layer's rows r, r + 1
+        const int r = Y0 + 2 * k + 20 - 4 * L;         // this layer's rows r, r + 1
         auto rd = [&](int row, int d, int mm) {
-            return fs_read_frag(h_in + (uint32_t)(fs_slot6(row, Y0) * FS_ROWB) + ha[d] + mm * 1024);
+            return fs_read_frag(h_in + (uint32_t)(fs_slot(row, Y0) * FS_ROWB) + ha[d] + mm * 1024);
         };
 
-        // ---- loads: row r+1 (slot C) and the role's inputs
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int mm = 0; mm < 2; mm++) fr[Cs][d][mm] = rd(r + 1, d, mm);
-        const int r0 = Y0 + 2 * k + 15;                // first layer's rows (R_FIRST)
+        const int r0 = Y0 + 2 * k + 19;                // first layer's rows (R_FIRST)
         const int yg = Y0 + 2 * k + (wave - 4);        // output row (R_GATHER)
-        u32x4 fdw[2];
-        bf16x8 fw0[2];
-        f32x4 fb0[2];
-        float gv[9];
         bf16x8 w6;
-        if constexpr (RL == R_FIRST) {
-#pragma unroll
-            for (int rw = 0; rw < 2; rw++) {
-                const uint32_t base = FS_IMGW_OFF + (uint32_t)(img_slot(r0 + rw - 1) * FS_IMG_PITCH * 4);
-#pragma unroll
-                for (int q = 0; q < 4; q++) fdw[rw][q] = fs_read_u32(base + ioff[q]);
-            }
-#pragma unroll
-            for (int nt = 0; nt < 2; nt++) {
-                fw0[nt] = fs_read_frag(FS_W0_OFF + 1024 * nt + 16 * lane);
-                fb0[nt] = fs_read_f4(FS_CB_OFF + 64 * nt + 16 * kq);
-            }
-        } else if constexpr (RL == R_GATHER) {
-#pragma unroll
-            for (int t = 0; t < 9; t++) {
-                const int dy = t / 3 - 1, dx = t % 3 - 1;
-                gv[t] = fs_read_f32(FS_D_OFF + (uint32_t)(fs_slot6(yg + dy, Y0) * FS_D_ROWB) +
-                                    (uint32_t)((t * FS_D_PITCH + lane + 1 + dx) * 4));
-            }
-        } else {
-            w6 = fs_read_frag(FS_W6_OFF + 16 * lane);
-        }
+        if constexpr (RL == R_LAST) w6 = fs_read_frag(FS_W6_OFF + 16 * lane);
 
         // epilogue of one row: BF16 + ReLU (zero outside the image), then the
         // ring write, or for layer 4 the last layer's products
@@ -1136,14 +1141,14 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                     }
                 }
                 if constexpr (RL != R_LAST) {
-                    const uint32_t wb = h_out + (uint32_t)(fs_slot6(row, Y0) * FS_ROWB) + mm * 1024;
+                    const uint32_t wb = h_out + (uint32_t)(fs_slot(row, Y0) * FS_ROWB) + mm * 1024;
                     fs_write_b64(wb + hw[0], p[0]);
                     fs_write_b64(wb + hw[1], p[1]);
                 } else {
                     // D[t][x] = sum over the 32 channels of W6[t] * act[x]
                     const bf16x8 bv = __builtin_bit_cast(bf16x8, (u32x4){ p[0][0], p[0][1], p[1][0], p[1][1] });
                     const f32x4 dsum = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w6, bv, (f32x4){ 0.f, 0.f, 0.f, 0.f }, 0, 0, 0);
-                    const uint32_t db = FS_D_OFF + (uint32_t)(fs_slot6(row, Y0) * FS_D_ROWB) +
+                    const uint32_t db = FS_D_OFF + (uint32_t)(fs_dslot(row, Y0) * FS_D_ROWB) +
                                         (uint32_t)(((4 * kq) * FS_D_PITCH + 16 * (m0 + mm) + i16) * 4);
 #pragma unroll
                     for (int q = 0; q < 4; q++) fs_write_f32(db + q * FS_D_PITCH * 4, dsum[q]);
@@ -1151,10 +1156,16 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
             }
         };
 
-        // ---- the MFMAs, ordered so that those on the carried rows (r-1 in
-        // slot A, r in slot B) come first: 36 of the 72 run before row r+1
-        // (slot C, loaded at the step's start) is needed.
-        f32x4 acc0[2][2], acc1[2][2];
+        // ---- the MFMAs.  Each accumulator takes its taps in order 0..8 from
+        // the bias, as the per-layer kernel does.  Order: row r's taps on row
+        // r-1 (slot A) with the previous step's row r-1 epilogue beside them
+        // (it reads acc1 before acc1 is reused); row r's taps on row r (slot
+        // B) while row r+1 loads into slot C; row r+1's taps on row r; row
+        // r+2 loads into slot A (row r-1 is consumed) during the role's
+        // work; row r's taps on row r+1; row r+1's on rows r+1 and r+2, with
+        // row r's epilogue beside the last 24 MFMAs.  Loading each row only
+        // when a slot frees keeps two row slots live, not three.
+        f32x4 bias[2];
         auto mf = [&](f32x4 (&acc)[2][2], int s, const bf16x8 (&f)[3][2], int d) __attribute__((always_inline)) {
 #pragma unroll
             for (int mm = 0; mm < 2; mm++)
@@ -1163,24 +1174,58 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                     acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         wA[s][nt], f[d][mm], s == 0 ? bias[nt] : acc[mm][nt], 0, 0, 0);
         };
-        // row r, dy = -1 (row r-1, slot A)
+        f32x4 acc0[2][2];
+        read_bias(bias);
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc0, d, fr[A], d);
+        finish(acc1, r - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        // row r+1 into slot C
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int mm = 0; mm < 2; mm++) fr[Cs][d][mm] = rd(r + 1, d, mm);
+#pragma unroll
+        for (int d = 0; d < 3; d++) mf(acc0, 3 + d, fr[B], d);
+        // the role's inputs
+        u32x4 fdw[2];
+        bf16x8 fw0[2];
+        f32x4 fb0[2];
+        float gv[9];
+        uint8_t px = 0;
+        if constexpr (RL == R_FIRST) {
+#pragma unroll
+            for (int rw = 0; rw < 2; rw++) {
+                const uint32_t base = FS_IMGW_OFF + (uint32_t)(img_slot(r0 + rw - 1) * FS_IMG_PITCH * 4);
+#pragma unroll
+                for (int q = 0; q < 4; q++) fdw[rw][q] = fs_read_u32(base + ioff[q]);
+            }
+#pragma unroll
+            for (int nt = 0; nt < 2; nt++) {
+                fw0[nt] = fs_read_frag(FS_W0_OFF + 1024 * nt + 16 * lane);
+                fb0[nt] = fs_read_f4(FS_CB_OFF + 64 * nt + 16 * kq);
+            }
+        } else if constexpr (RL == R_GATHER) {
+#pragma unroll
+            for (int t = 0; t < 9; t++) {
+                const int dy = t / 3 - 1, dx = t % 3 - 1;
+                gv[t] = fs_read_f32(FS_D_OFF + (uint32_t)(fs_dslot(yg + dy, Y0) * FS_D_ROWB) +
+                                    (uint32_t)((t * FS_D_PITCH + lane + 1 + dx) * 4));
+            }
+            px = fs_read_u8(FS_IMGU_OFF + img_slot(yg) * FS_IMG_PITCH + lane + 6);
+        }
+
+        read_bias(bias);
+#pragma unroll
+        for (int d = 0; d < 3; d++) mf(acc1, d, fr[B], d);
         __builtin_amdgcn_sched_barrier(0);
         // row r+2 into slot A (row r-1 is consumed)
 #pragma unroll
         for (int d = 0; d < 3; d++)
 #pragma unroll
             for (int mm = 0; mm < 2; mm++) fr[A][d][mm] = rd(r + 2, d, mm);
-        // row r+1, dy = -1 and row r, dy = 0 (row r, slot B)
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            mf(acc1, d, fr[B], d);
-            mf(acc0, 3 + d, fr[B], d);
-        }
 
-        // ---- the role's own work (independent of the rows above)
-        float gsum = 0.0f;
+        // ---- the role's own work (independent of this step's MFMAs)
         if constexpr (RL == R_FIRST) {
             const int c0 = 16 * wave + i16;
 #pragma unroll
@@ -1196,67 +1241,39 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                         p[nt] = in ? p[nt] : (u32x2){ 0u, 0u };
                     }
                 }
-                const uint32_t wb = (uint32_t)(fs_slot6(row, Y0) * FS_ROWB);
+                const uint32_t wb = (uint32_t)(fs_slot(row, Y0) * FS_ROWB);
                 fs_write_b64(wb + fs_col(c0, kq >> 1) + (kq & 1) * 8, p[0]);
                 fs_write_b64(wb + fs_col(c0, 2 + (kq >> 1)) + (kq & 1) * 8, p[1]);
             }
         } else if constexpr (RL == R_GATHER) {
-            gsum = gv[0];
+            // image rows for the first layer (see the streaming note above),
+            // before the output store: the wait for the bytes loaded a step
+            // ago then never waits for this step's store
+            const int rs = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
+            put_img(rs + prr0, pcc0, pb0);
+            if (ptid + 128 < FS_ROW2) put_img(rs + prr1, pcc1, pb1);
+            pb0 = fs_load_img(a, rs + 2 + prr0, img_x + pcc0);
+            pb1 = fs_load_img(a, rs + 2 + prr1, img_x + pcc1);
+            // output row: residual + the products summed in tap order
+            float gsum = gv[0];
 #pragma unroll
             for (int t = 1; t < 9; t++) gsum = gsum + gv[t];
-        }
-        __builtin_amdgcn_sched_barrier(0);
-
-        // ---- row r, dy = 1 and row r+1, dy = 0 (row r+1, slot C); row r+1,
-        // dy = 1 (row r+2, slot A); row r's epilogue meanwhile
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            mf(acc0, 6 + d, fr[Cs], d);
-            mf(acc1, 3 + d, fr[Cs], d);
-        }
-#pragma unroll
-        for (int d = 0; d < 3; d++) mf(acc1, 6 + d, fr[A], d);
-        finish(acc0, r);
-        finish(acc1, r + 1);
-
-        // ---- output row (R_GATHER): residual + the summed products
-        if constexpr (RL == R_GATHER) {
             const int gx = x0 + lane;
             if (k >= 0 && yg < Y1 && lane < FS_W && gx < W) {
-                const uint8_t px = fs_read_u8(FS_IMGU_OFF + img_slot(yg) * FS_IMG_PITCH + lane + 6);
-                const float v = (float)px / 255.0f + (gsum + b6);
+                const float v = fs_read_f32(FS_RES_OFF + 4 * px) + (gsum + b6);
                 const float q = rintf(v * 255.0f);
                 a.out[(int64_t)yg * a.out_pitch + gx] = (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
             }
-            const int phase = (k - FS_KBEGIN) & 7;
-            if (phase == 7) {
-                const int rp = img_base + FS_PRE + 2 * (k - 7 - FS_KBEGIN);   // the rows loaded at phase 0
-#pragma unroll
-                for (int j = 0; j < FS_PF; j++) {
-                    int e = ptid + 128 * j;
-                    asm volatile("" : "+v"(e));
-                    if (e < FS_CHUNK_BYTES) {
-                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-                        put_img(rp + rr, cc, (uint8_t)(pf[j >> 2] >> (8 * (j & 3))));
-                    }
-                }
-            }
-            if (phase == 0) {
-                // image rows img_base + FS_PRE + 16 c .. + 15 (c = (k - FS_KBEGIN) / 8), written at phase 7
-                const int rp = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
-#pragma unroll
-                for (int j = 0; j < (FS_PF + 3) / 4; j++) pf[j] = 0;
-#pragma unroll
-                for (int j = 0; j < FS_PF; j++) {
-                    int e = ptid + 128 * j;
-                    asm volatile("" : "+v"(e));     // recomputed here, not held in registers across the loop
-                    if (e < FS_CHUNK_BYTES) {
-                        const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
-                        pf[j >> 2] |= (uint32_t)fs_load_img(a, rp + rr, img_x + cc) << (8 * (j & 3));
-                    }
-                }
-            }
         }
+        __builtin_amdgcn_sched_barrier(0);
+
+#pragma unroll
+        for (int d = 0; d < 3; d++) mf(acc0, 6 + d, fr[Cs], d);
+#pragma unroll
+        for (int d = 0; d < 3; d++) mf(acc1, 3 + d, fr[Cs], d);
+        finish(acc0, r);
+#pragma unroll
+        for (int d = 0; d < 3; d++) mf(acc1, 6 + d, fr[A], d);
         fs_step_barrier();
     };
 
