@@ -965,7 +965,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const int H = a.H, W = a.W;
     const int img_base = Y0 - 6;              // image row of ring row 0
     const int img_x = x0 - 6;                 // image column of ring column 0
-    const int ptid = tid - 256;               // image streaming thread (waves 4, 5)
+    const int ptid = tid - 384;               // image streaming thread (waves 6, 7)
 
     auto img_slot = [&](int row) { return (row - img_base) & (FS_IMG_ROWS - 1); };
 
@@ -1041,14 +1041,19 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         *(__attribute__((address_space(3))) bf16x8 *)(size_t)(FS_W6_OFF + 16 * l) = v;
     }
     fs_step_barrier();
-    auto put_img = [&](int row, int cc, uint8_t u) {
+    // image byte u of ring column cc (image column img_x + cc, inside the
+    // image when col_in) into both rings
+    auto put_img_in = [&](int row, int cc, uint8_t u, bool col_in) {
         const int sl = img_slot(row);
-        const int gx = img_x + cc;
-        const bool in = row >= 0 && row < H && gx >= 0 && gx < W;
+        const bool in = row >= 0 && row < H && col_in;
         const uint32_t wv = in ? fs_read_u32(FS_LUT_OFF + 4 * u) : 0u;
         fs_write_u8(FS_IMGU_OFF + sl * FS_IMG_PITCH + cc, u);
         fs_write_u32(FS_IMGW_OFF + (sl * FS_IMG_PITCH + cc) * 4, wv);
         if (sl < FS_IMG_DUP) fs_write_u32(FS_IMGW_OFF + ((sl + FS_IMG_ROWS) * FS_IMG_PITCH + cc) * 4, wv);
+    };
+    auto put_img = [&](int row, int cc, uint8_t u) {
+        const int gx = img_x + cc;
+        put_img_in(row, cc, u, gx >= 0 && gx < W);
     };
     for (int e = tid; e < FS_PRE * FS_IMG_W; e += FS_WAVES * 64) {
         const int rr = e / FS_IMG_W, cc = e - rr * FS_IMG_W;
@@ -1083,7 +1088,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 #pragma unroll
             for (int nt = 0; nt < 2; nt++) acc1[mm][nt] = b[nt];
     }
-    // Image streaming (waves 4, 5): at step s = k - FS_KBEGIN the first layer
+    // Image streaming (waves 6, 7): at step s = k - FS_KBEGIN the first layer
     // reads image rows img_base + 2s .. + 2s + 3.  Step s writes rows
     // img_base + 2s + 4 and + 5 into the ring (visible from step s + 1) from
     // the bytes loaded at step s - 1, then loads rows + 6 and + 7.  Thread
@@ -1093,8 +1098,11 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const int pe0 = ptid, pe1 = min(ptid + 128, FS_ROW2 - 1);
     const int prr0 = pe0 / FS_IMG_W, pcc0 = pe0 - prr0 * FS_IMG_W;
     const int prr1 = pe1 / FS_IMG_W, pcc1 = pe1 - prr1 * FS_IMG_W;
+    // the streamed columns, clamped into the image, and whether they lie in it
+    const int pcol0 = min(max(img_x + pcc0, 0), W - 1), pcol1 = min(max(img_x + pcc1, 0), W - 1);
+    const bool pin0 = img_x + pcc0 >= 0 && img_x + pcc0 < W, pin1 = img_x + pcc1 >= 0 && img_x + pcc1 < W;
     uint8_t pb0 = 0, pb1 = 0;
-    if (tid >= 256 && tid < 384) {
+    if (tid >= 384) {
         pb0 = fs_load_img(a, img_base + FS_PRE + prr0, img_x + pcc0);
         pb1 = fs_load_img(a, img_base + FS_PRE + prr1, img_x + pcc1);
     }
@@ -1245,15 +1253,17 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                 fs_write_b64(wb + fs_col(c0, kq >> 1) + (kq & 1) * 8, p[0]);
                 fs_write_b64(wb + fs_col(c0, 2 + (kq >> 1)) + (kq & 1) * 8, p[1]);
             }
-        } else if constexpr (RL == R_GATHER) {
-            // image rows for the first layer (see the streaming note above),
-            // before the output store: the wait for the bytes loaded a step
-            // ago then never waits for this step's store
+        } else if constexpr (RL == R_LAST) {
+            // image rows for the first layer (see the streaming note above).
+            // The rows loaded here are >= Y0 >= 0; only the bottom edge clamps.
             const int rs = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
-            put_img(rs + prr0, pcc0, pb0);
-            if (ptid + 128 < FS_ROW2) put_img(rs + prr1, pcc1, pb1);
-            pb0 = fs_load_img(a, rs + 2 + prr0, img_x + pcc0);
-            pb1 = fs_load_img(a, rs + 2 + prr1, img_x + pcc1);
+            put_img_in(rs + prr0, pcc0, pb0, pin0);
+            if (ptid + 128 < FS_ROW2) put_img_in(rs + prr1, pcc1, pb1, pin1);
+            const uint8_t *ra = a.img + (int64_t)min(rs + 2, H - 1) * a.img_pitch;
+            const uint8_t *rb = a.img + (int64_t)min(rs + 3, H - 1) * a.img_pitch;
+            pb0 = (prr0 ? rb : ra)[pcol0];
+            pb1 = (prr1 ? rb : ra)[pcol1];
+        } else if constexpr (RL == R_GATHER) {
             // output row: residual + the products summed in tap order
             float gsum = gv[0];
 #pragma unroll
