@@ -922,13 +922,15 @@ __device__ __forceinline__ u32x2 fs_act4(f32x4 acc)
 }
 
 // Byte offset, within a ring row, of 16-byte chunk `chunk` of column x.
-// The chunk is XORed with (b1 ^ b2, b2 ^ b3) of x: the same logical chunk
-// of 16 consecutive pixels then lands on 16 distinct 16-byte bank groups of
-// a 256-byte bank line, and of any 8 consecutive pixels (x even first) on 8
-// distinct groups of a 128-byte line; it depends on x mod 16 only.
+// The chunk is XORed with bits 1..2 of x.  Measured on gfx950 (256
+// workgroups of 8 waves): the fragment read (ds_read_b128 of chunk kq of 16
+// consecutive columns starting at any column) and the epilogue's ds_write_b64
+// then run at the conflict-free rate; XORing with (b1^b2, b2^b3) of x, or
+// with no swizzle, made the reads starting at an odd column (or all writes)
+// two to four times slower.
 __device__ __forceinline__ uint32_t fs_col(int x, int chunk)
 {
-    const int sw = (((x >> 1) ^ (x >> 2)) & 1) | ((((x >> 2) ^ (x >> 3)) & 1) << 1);
+    const int sw = (x >> 1) & 3;
     return (uint32_t)(x * 64 + ((chunk ^ sw) << 4));
 }
 
