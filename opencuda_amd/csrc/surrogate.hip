@@ -1184,12 +1184,18 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                     acc[mm][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         wA[s][nt], f[d][mm], s == 0 ? bias[nt] : acc[mm][nt], 0, 0, 0);
         };
+        // Wave priority falls 3, 2, 1, 0 through the step's four phases, so
+        // of a SIMD's two waves the one behind issues first and the two
+        // reach the barrier together (the arbiter otherwise favours the
+        // older wave, and the younger one ends the step alone).
+        __builtin_amdgcn_s_setprio(3);
         f32x4 acc0[2][2];
         read_bias(bias);
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc0, d, fr[A], d);
         finish(acc1, r - 1);
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(2);
         // row r+1 into slot C
 #pragma unroll
         for (int d = 0; d < 3; d++)
@@ -1229,6 +1235,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc1, d, fr[B], d);
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(1);
         // row r+2 into slot A (row r-1 is consumed)
 #pragma unroll
         for (int d = 0; d < 3; d++)
@@ -1278,6 +1285,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
             }
         }
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
 
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc0, 6 + d, fr[Cs], d);
