@@ -872,9 +872,8 @@ constexpr int FS_W6_OFF = FS_W0_OFF + 2 * 64 * 16;                     // last l
 constexpr int FS_RES_OFF = FS_W6_OFF + 64 * 16;                        // residual u / 255 (FP32), u = 0..255
 constexpr int FS_LDS = FS_RES_OFF + 256 * 4;
 constexpr int FS_PRE = 4;                 // image rows written before the first step
-constexpr int FS_ROW2 = 2 * FS_IMG_W;     // bytes of the two image rows a step streams
 constexpr int FS_KBEGIN = -12;            // first step: layer 0 computes rows Y0-5, Y0-4
-static_assert(FS_ROW2 <= 2 * 128, "two image rows fit two bytes per thread of waves 4, 5");
+static_assert(FS_IMG_W <= 128, "an image row fits one byte per lane of two waves");
 static_assert(FS_LDS <= 160 * 1024, "LDS");
 static_assert(FS_IMG_PITCH >= FS_IMG_W, "image ring rows");
 
@@ -967,7 +966,6 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     const int H = a.H, W = a.W;
     const int img_base = Y0 - 6;              // image row of ring row 0
     const int img_x = x0 - 6;                 // image column of ring column 0
-    const int ptid = tid - 384;               // image streaming thread (waves 6, 7)
 
     auto img_slot = [&](int row) { return (row - img_base) & (FS_IMG_ROWS - 1); };
 
@@ -1090,24 +1088,20 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 #pragma unroll
             for (int nt = 0; nt < 2; nt++) acc1[mm][nt] = b[nt];
     }
-    // Image streaming (waves 6, 7): at step s = k - FS_KBEGIN the first layer
+    // Image streaming (waves 4..7): at step s = k - FS_KBEGIN the first layer
     // reads image rows img_base + 2s .. + 2s + 3.  Step s writes rows
     // img_base + 2s + 4 and + 5 into the ring (visible from step s + 1) from
-    // the bytes loaded at step s - 1, then loads rows + 6 and + 7.  Thread
-    // ptid streams bytes ptid and ptid + 128 of the two rows (the second
-    // only for ptid < FS_ROW2 - 128; the others load a valid byte and drop
-    // it).
-    const int pe0 = ptid, pe1 = min(ptid + 128, FS_ROW2 - 1);
-    const int prr0 = pe0 / FS_IMG_W, pcc0 = pe0 - prr0 * FS_IMG_W;
-    const int prr1 = pe1 / FS_IMG_W, pcc1 = pe1 - prr1 * FS_IMG_W;
-    // the streamed columns, clamped into the image, and whether they lie in it
-    const int pcol0 = min(max(img_x + pcc0, 0), W - 1), pcol1 = min(max(img_x + pcc1, 0), W - 1);
-    const bool pin0 = img_x + pcc0 >= 0 && img_x + pcc0 < W, pin1 = img_x + pcc1 >= 0 && img_x + pcc1 < W;
-    uint8_t pb0 = 0, pb1 = 0;
-    if (tid >= 384) {
-        pb0 = fs_load_img(a, img_base + FS_PRE + prr0, img_x + pcc0);
-        pb1 = fs_load_img(a, img_base + FS_PRE + prr1, img_x + pcc1);
-    }
+    // the bytes loaded at step s - 1, then loads rows + 6 and + 7: waves 4, 5
+    // the first of each pair, waves 6, 7 the second, one byte per lane
+    // (lanes past the row's FS_IMG_W bytes load a valid byte and drop it).
+    const int sq = tid - 256;                 // waves 4..7
+    const int prow = sq >> 7;                 // waves 4, 5: the first row; 6, 7: the second
+    const int pcc = min(sq & 127, FS_IMG_W - 1);
+    const bool pact = (sq & 127) < FS_IMG_W;
+    const int pcol = min(max(img_x + pcc, 0), W - 1);
+    const bool pin = img_x + pcc >= 0 && img_x + pcc < W;
+    uint8_t pb = 0;
+    if (tid >= 256) pb = fs_load_img(a, img_base + FS_PRE + prow, img_x + pcc);
 
     // Every layer computes its two rows at every step, needed or not: a row
     // outside a layer's needed range may hold garbage, but a needed row reads
@@ -1263,16 +1257,13 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                 fs_write_b64(wb + fs_col(c0, 2 + (kq >> 1)) + (kq & 1) * 8, p[1]);
             }
         } else if constexpr (RL == R_LAST) {
-            // image rows for the first layer (see the streaming note above).
-            // The rows loaded here are >= Y0 >= 0; only the bottom edge clamps.
             const int rs = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
-            put_img_in(rs + prr0, pcc0, pb0, pin0);
-            if (ptid + 128 < FS_ROW2) put_img_in(rs + prr1, pcc1, pb1, pin1);
-            const uint8_t *ra = a.img + (int64_t)min(rs + 2, H - 1) * a.img_pitch;
-            const uint8_t *rb = a.img + (int64_t)min(rs + 3, H - 1) * a.img_pitch;
-            pb0 = (prr0 ? rb : ra)[pcol0];
-            pb1 = (prr1 ? rb : ra)[pcol1];
+            if (pact) put_img_in(rs + prow, pcc, pb, pin);
+            pb = (a.img + (int64_t)min(rs + 2 + prow, H - 1) * a.img_pitch)[pcol];
         } else if constexpr (RL == R_GATHER) {
+            const int rs = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
+            if (pact) put_img_in(rs + prow, pcc, pb, pin);
+            pb = (a.img + (int64_t)min(rs + 2 + prow, H - 1) * a.img_pitch)[pcol];
             // output row: residual + the products summed in tap order
             float gsum = gv[0];
 #pragma unroll
