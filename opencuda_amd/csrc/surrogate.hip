@@ -1403,8 +1403,37 @@ int launch_conv(const ConvArgs &a, hipStream_t s)
     return kOk;
 }
 
+// Band height of the fused kernel.  A band of B rows costs ceil(B / 2) steps
+// plus -FS_KBEGIN warm-up steps (and a few steps' worth of set-up), and the
+// grid runs in rounds of one workgroup per CU, so the launch takes about
+// ceil(workgroups / CUs) x that.  Images whose 256-row bands fill the
+// device get the band height that minimises this; smaller ones keep
+// 256-row bands (more workgroups for a device they do not fill, and less
+// warm-up work when several images share the device).
+int fused_band(int H, int strips)
+{
+    const int base = std::min(H, 256);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return base;
+    auto makespan = [&](int B) {
+        const int64_t bands = (H + B - 1) / B;
+        const int64_t rounds = ((int64_t)strips * bands + cus - 1) / cus;
+        return rounds * ((B + 1) / 2 - FS_KBEGIN + 3);
+    };
+    if ((int64_t)strips * ((H + 255) / 256) < cus) return base;
+    int best = base;
+    int64_t best_cost = makespan(base);
+    for (int B = 258; B <= H; B += 2) {
+        const int64_t c = makespan(B);
+        if (c < best_cost) { best_cost = c; best = B; }
+    }
+    return best;
+}
+
 // Fused inference of one image (u8 in, u8 out): strips of FS_W columns x
-// bands of up to 256 rows, one workgroup of FS_WAVES waves each.
+// bands of fused_band rows, one workgroup of FS_WAVES waves each.
 int launch_fused(Net *net, const uint8_t *img, int64_t img_pitch, uint8_t *out, int64_t out_pitch,
                  int H, int W, float sigma_range, hipStream_t s)
 {
@@ -1418,8 +1447,8 @@ int launch_fused(Net *net, const uint8_t *img, int64_t img_pitch, uint8_t *out, 
     a.out_pitch = out_pitch;
     a.H = H;
     a.W = W;
-    a.band = std::min(H, 256);
     a.strips = (W + FS_W - 1) / FS_W;
+    a.band = fused_band(H, a.strips);
     a.sr = sigma_range / 100.0f;
     for (int l = 0; l < NLAYERS; l++) {
         a.fw[l] = net->fw[l];
