@@ -834,18 +834,19 @@ __global__ void pack_kernel(const float *params, PackTables pk)
 // last layer, computed per pixel as D[t][x] = sum_ci W6[t][ci] * act[x][ci]
 // for the nine taps t (one MFMA with the taps as rows), written to a ring of
 // FP32 products; waves 4,5 sum each output pixel's nine products in tap
-// order, add the residual and store, and stream the image: each step they
-// write two image rows loaded into registers the step before and load the
-// next two, so a load has a whole step to arrive.
+// order, add the residual and store.  Waves 4..7 stream the image: each
+// step they write two image rows loaded into registers the step before and
+// load the next two, so a load has a whole step to arrive.
 //
 // The MFMAs compute the transposed product (weights as the A operand,
 // activations as B), so a lane ends with four consecutive channels of one
 // pixel and stores them with one 8-byte write.  Bias is the accumulator's
 // initial value; the epilogue rounds to BF16 and then applies the ReLU to
 // the BF16 bits.  Activations are 64-byte NHWC pixels in 64-pixel ring
-// rows; the 16-byte chunks of a pixel are permuted by a function of its
-// column mod 16 (fs_col), so the consecutive pixels one ds_read_b128
-// gathers fall into distinct banks.
+// rows; the 16-byte chunks of a pixel are permuted by bits 1..2 of its
+// column (fs_col), so the ds_read_b128 and ds_write_b64 of a step run at
+// the conflict-free rate.  Each wave lowers its priority through the step
+// (the ladder in `step`) so that a SIMD's two waves stay in step.
 //
 // Operands, their order and every rounding are those of conv_kernel, so the
 // output equals the per-layer path's bit for bit.
