@@ -988,14 +988,12 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     uint32_t hw[2];
 #pragma unroll
     for (int nt = 0; nt < 2; nt++) hw[nt] = fs_col(16 * m0 + i16, 2 * nt + (kq >> 1)) + (kq & 1) * 8;
-    // image columns of layer L's two M-tiles; whole M-tiles inside the image need no mask
+    // whether this lane's column of layer L's two M-tiles lies in the image
     bool hcol[2];
-    bool hall[2];
 #pragma unroll
     for (int mm = 0; mm < 2; mm++) {
         const int c0 = x0 - (5 - L) + 16 * (m0 + mm);
         hcol[mm] = c0 + i16 >= 0 && c0 + i16 < W;
-        hall[mm] = c0 >= 0 && c0 + 15 < W;
     }
     const uint32_t h_in = (uint32_t)((L - 1) * FS_RING * FS_ROWB);
     const uint32_t h_out = (uint32_t)(L * FS_RING * FS_ROWB);     // unused by layer 4
@@ -1006,7 +1004,7 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     // weights are zero: they read the centre tap's (finite, >= 0) dword, so
     // their products are +0, as the per-layer kernel's zero operands give
     uint32_t ioff[4] = {};
-    bool fcol = false, fall = false;
+    bool fcol = false;
     if (first_w) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -1016,7 +1014,6 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         }
         const int c0 = x0 - 5 + 16 * wave;
         fcol = c0 + i16 >= 0 && c0 + i16 < W;
-        fall = c0 >= 0 && c0 + 15 < W;
     }
     const float b6 = a.bias[NLAYERS - 1][0];
 
@@ -1349,10 +1346,10 @@ Net *net_of(bf_sur h) { return reinterpret_cast<Net *>(h); }
 int ensure_ws(Net *net, size_t pix)
 {
     if (pix <= net->ws_pix) return kOk;
-    for (auto &p : net->act) { hipFree(p); p = nullptr; }
-    for (auto &p : net->dbuf) { hipFree(p); p = nullptr; }
-    hipFree(net->g6);
-    hipFree(net->pred);
+    for (auto &p : net->act) { (void)hipFree(p); p = nullptr; }
+    for (auto &p : net->dbuf) { (void)hipFree(p); p = nullptr; }
+    (void)hipFree(net->g6);
+    (void)hipFree(net->pred);
     net->g6 = nullptr;
     net->pred = nullptr;
     net->ws_pix = 0;
@@ -1584,19 +1581,19 @@ int backward(Net *net, const uint8_t *img, const uint8_t *target, const uint8_t 
 
 void free_net(Net *net)
 {
-    hipFree(net->params);
-    hipFree(net->m);
-    hipFree(net->v);
-    hipFree(net->grad);
-    for (auto p : net->fw) hipFree(p);
-    for (auto p : net->bw) hipFree(p);
-    for (auto p : net->act) hipFree(p);
-    for (auto p : net->dbuf) hipFree(p);
-    hipFree(net->g6);
-    hipFree(net->pred);
-    hipFree(net->partial);
-    hipFree(net->list);
-    hipFree(net->count);
+    (void)hipFree(net->params);
+    (void)hipFree(net->m);
+    (void)hipFree(net->v);
+    (void)hipFree(net->grad);
+    for (auto p : net->fw) (void)hipFree(p);
+    for (auto p : net->bw) (void)hipFree(p);
+    for (auto p : net->act) (void)hipFree(p);
+    for (auto p : net->dbuf) (void)hipFree(p);
+    (void)hipFree(net->g6);
+    (void)hipFree(net->pred);
+    (void)hipFree(net->partial);
+    (void)hipFree(net->list);
+    (void)hipFree(net->count);
     delete net;
 }
 
@@ -1826,7 +1823,7 @@ static int sur_filter(Net *net, const bf_image_view *in, const bf_image_view *ou
     if ((rc = infer_view(net, in, out, sigma_range, s, &Wi, &Hi))) return rc;
     if (!given) {
         if (nt > net->fb_tiles) {
-            hipFree(net->list);
+            (void)hipFree(net->list);
             net->list = nullptr;
             net->fb_tiles = 0;
             BF_HIP(hipMalloc(&net->list, nt * 4));
