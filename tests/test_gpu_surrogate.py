@@ -219,7 +219,7 @@ def test_check_flags_exactly_the_sampled_tiles(device, radius, mode, table):
     ((530, 613), None, 40.0),              # three bands of 256 rows
     ((40, 1000), (3, 5, 998, 37), 15.0),   # pitched ROI view, fewer rows than a band
     ((13, 13), None, 30.0),                # smaller than one strip
-    ((1200, 3000), None, 22.0),            # fills a 256-CU device: bands of 300 rows
+    ((1111, 3200), None, 22.0),            # fills a 256-CU device: bands of 278 rows, the last 277
 ])
 def test_fused_inference_equals_per_layer_path(device, shape, roi, sr):
     import torch
