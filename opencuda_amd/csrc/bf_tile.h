@@ -278,7 +278,11 @@ __device__ __forceinline__ void tile_ring(const DevJob &job, int X0, int Y0, int
     float t0 = 0.0f, t1 = 0.0f, t2 = 0.0f, t3 = 0.0f;
     float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
     constexpr int MINB = ring_min_offset<R, MODE, G::TCOLS>() * 4;
-    const uint32_t tbase = lane + (uint32_t)MINB;   // >= 0: the halo covers every tap
+    uint32_t tbase = lane + (uint32_t)MINB;         // >= 0: the halo covers every tap
+    // Kept opaque: folded into `lane`, the taps above and left of the pixel
+    // would need negative offsets, which ds_read's unsigned offset field
+    // cannot hold (each such fetch then costs a v_add_u32).
+    asm volatile("" : "+v"(tbase));
     uint32_t wp[2][4] = {};                         // STD: side-column windows (right, left)
     float wf[2][4] = {};
     // Gathers address W[row][d] as (4d + LUT_BASE) + row*1024; rows past 63
