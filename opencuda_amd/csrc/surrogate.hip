@@ -948,8 +948,7 @@ __device__ __forceinline__ uint8_t fs_load_img(const FusedArgs &a, int row, int 
     return a.img[(int64_t)row * a.img_pitch + col];
 }
 
-// Slot of a row in an activation ring and in the product ring.
-__device__ __forceinline__ int fs_slot(int row, int Y0) { return (row - Y0) & (FS_RING - 1); }
+// Slot of a row in the product ring.
 __device__ __forceinline__ int fs_dslot(int row, int Y0) { return (row - Y0) & (FS_DRING - 1); }
 
 template <int I> using fs_ic = std::integral_constant<int, I>;
@@ -997,6 +996,14 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     }
     const uint32_t h_in = (uint32_t)((L - 1) * FS_RING * FS_ROWB);
     const uint32_t h_out = (uint32_t)(L * FS_RING * FS_ROWB);     // unused by layer 4
+    // per-lane ring addresses without the row slot: the slot of every row a
+    // step touches is a compile-time constant of the step's parity (RP), so
+    // it goes into the ds_read/ds_write immediate
+    uint32_t hb[3], hwb[2];
+#pragma unroll
+    for (int d = 0; d < 3; d++) hb[d] = h_in + ha[d];
+#pragma unroll
+    for (int nt = 0; nt < 2; nt++) hwb[nt] = h_out + hw[nt];
 
     // ---- role data
     const bool first_w = wave < 4;            // first layer, M-tile `wave`
@@ -1005,6 +1012,9 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     // their products are +0, as the per-layer kernel's zero operands give
     uint32_t ioff[4] = {};
     bool fcol = false;
+    // first layer's ring addresses of this lane's channels, without the slot
+    const uint32_t fw1a = fs_col(16 * wave + i16, kq >> 1) + (kq & 1) * 8;
+    const uint32_t fw1b = fs_col(16 * wave + i16, 2 + (kq >> 1)) + (kq & 1) * 8;
     if (first_w) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -1113,13 +1123,15 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
     // 7, layer 4): the last layer's products.  MASKED: zero every activation
     // outside the image (workgroups at an image border).
     constexpr int R_FIRST = 0, R_GATHER = 1, R_LAST = 2;
-    auto step = [&](auto ROLE, auto MASKED, auto SA, auto SB, auto SC, int k) __attribute__((always_inline)) {
+    auto step = [&](auto ROLE, auto MASKED, auto SA, auto SB, auto SC, auto RPH, int k) __attribute__((always_inline)) {
         constexpr int RL = decltype(ROLE)::value;
+        constexpr int RP = decltype(RPH)::value;   // k & 1: row r lies in ring slot 2 RP
         constexpr bool MK = decltype(MASKED)::value;
         constexpr int A = decltype(SA)::value, B = decltype(SB)::value, Cs = decltype(SC)::value;
         const int r = Y0 + 2 * k + 20 - 4 * L;         // this layer's rows r, r + 1
-        auto rd = [&](int row, int d, int mm) {
-            return fs_read_frag(h_in + (uint32_t)(fs_slot(row, Y0) * FS_ROWB) + ha[d] + mm * 1024);
+        auto rd = [&](auto OFF, int d, int mm) {         // row r + OFF
+            constexpr uint32_t sl = (uint32_t)((2 * RP + decltype(OFF)::value) & (FS_RING - 1));
+            return fs_read_frag(hb[d] + sl * FS_ROWB + (uint32_t)(mm * 1024));
         };
 
         const int r0 = Y0 + 2 * k + 19;                // first layer's rows (R_FIRST)
@@ -1129,7 +1141,8 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 
         // epilogue of one row: BF16 + ReLU (zero outside the image), then the
         // ring write, or for layer 4 the last layer's products
-        auto finish = [&](const f32x4 (&acc)[2][2], int row) __attribute__((always_inline)) {
+        auto finish = [&](const f32x4 (&acc)[2][2], int row, auto SLOT) __attribute__((always_inline)) {
+            constexpr uint32_t sl = (uint32_t)decltype(SLOT)::value;
             const bool row_in = row >= 0 && row < H;
 #pragma unroll
             for (int mm = 0; mm < 2; mm++) {
@@ -1143,9 +1156,8 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                     }
                 }
                 if constexpr (RL != R_LAST) {
-                    const uint32_t wb = h_out + (uint32_t)(fs_slot(row, Y0) * FS_ROWB) + mm * 1024;
-                    fs_write_b64(wb + hw[0], p[0]);
-                    fs_write_b64(wb + hw[1], p[1]);
+                    fs_write_b64(hwb[0] + sl * FS_ROWB + (uint32_t)(mm * 1024), p[0]);
+                    fs_write_b64(hwb[1] + sl * FS_ROWB + (uint32_t)(mm * 1024), p[1]);
                 } else {
                     // D[t][x] = sum over the 32 channels of W6[t] * act[x]
                     const bf16x8 bv = __builtin_bit_cast(bf16x8, (u32x4){ p[0][0], p[0][1], p[1][0], p[1][1] });
@@ -1185,14 +1197,14 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         read_bias(bias);
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc0, d, fr[A], d);
-        finish(acc1, r - 1);
+        finish(acc1, r - 1, fs_ic<(2 * RP + 3) & (FS_RING - 1)>());
         __builtin_amdgcn_sched_barrier(0);
         __builtin_amdgcn_s_setprio(2);
         // row r+1 into slot C
 #pragma unroll
         for (int d = 0; d < 3; d++)
 #pragma unroll
-            for (int mm = 0; mm < 2; mm++) fr[Cs][d][mm] = rd(r + 1, d, mm);
+            for (int mm = 0; mm < 2; mm++) fr[Cs][d][mm] = rd(fs_ic<1>(), d, mm);
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc0, 3 + d, fr[B], d);
         // the role's inputs
@@ -1232,11 +1244,10 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
 #pragma unroll
         for (int d = 0; d < 3; d++)
 #pragma unroll
-            for (int mm = 0; mm < 2; mm++) fr[A][d][mm] = rd(r + 2, d, mm);
+            for (int mm = 0; mm < 2; mm++) fr[A][d][mm] = rd(fs_ic<2>(), d, mm);
 
         // ---- the role's own work (independent of this step's MFMAs)
         if constexpr (RL == R_FIRST) {
-            const int c0 = 16 * wave + i16;
 #pragma unroll
             for (int rw = 0; rw < 2; rw++) {
                 const int row = r0 + rw;
@@ -1250,9 +1261,9 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
                         p[nt] = in ? p[nt] : (u32x2){ 0u, 0u };
                     }
                 }
-                const uint32_t wb = (uint32_t)(fs_slot(row, Y0) * FS_ROWB);
-                fs_write_b64(wb + fs_col(c0, kq >> 1) + (kq & 1) * 8, p[0]);
-                fs_write_b64(wb + fs_col(c0, 2 + (kq >> 1)) + (kq & 1) * 8, p[1]);
+                const uint32_t wb = (uint32_t)(((2 * RP + 3 + rw) & (FS_RING - 1)) * FS_ROWB);
+                fs_write_b64(fw1a + wb, p[0]);
+                fs_write_b64(fw1b + wb, p[1]);
             }
         } else if constexpr (RL == R_LAST) {
             const int rs = img_base + FS_PRE + 2 * (k - FS_KBEGIN);
@@ -1280,19 +1291,26 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         for (int d = 0; d < 3; d++) mf(acc0, 6 + d, fr[Cs], d);
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc1, 3 + d, fr[Cs], d);
-        finish(acc0, r);
+        finish(acc0, r, fs_ic<(2 * RP) & (FS_RING - 1)>());
 #pragma unroll
         for (int d = 0; d < 3; d++) mf(acc1, 6 + d, fr[A], d);
         fs_step_barrier();
     };
 
     auto run = [&](auto ROLE, auto MASKED) __attribute__((always_inline)) {
-        for (int k = FS_KBEGIN; k < KE; k += 3) {
-            step(ROLE, MASKED, fs_ic<0>(), fs_ic<1>(), fs_ic<2>(), k);
+        static_assert((FS_KBEGIN & 1) == 0, "the unrolled steps start on an even k");
+        for (int k = FS_KBEGIN; k < KE; k += 6) {
+            step(ROLE, MASKED, fs_ic<0>(), fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), k);
             if (k + 1 >= KE) break;
-            step(ROLE, MASKED, fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), k + 1);
+            step(ROLE, MASKED, fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), fs_ic<1>(), k + 1);
             if (k + 2 >= KE) break;
-            step(ROLE, MASKED, fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), k + 2);
+            step(ROLE, MASKED, fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), fs_ic<0>(), k + 2);
+            if (k + 3 >= KE) break;
+            step(ROLE, MASKED, fs_ic<0>(), fs_ic<1>(), fs_ic<2>(), fs_ic<1>(), k + 3);
+            if (k + 4 >= KE) break;
+            step(ROLE, MASKED, fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), fs_ic<0>(), k + 4);
+            if (k + 5 >= KE) break;
+            step(ROLE, MASKED, fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), k + 5);
         }
     };
     // interior workgroup: every row and column a layer keeps lies in the image
