@@ -1297,34 +1297,51 @@ __global__ void __launch_bounds__(FS_WAVES * 64) fused_infer_kernel(FusedArgs a)
         fs_step_barrier();
     };
 
-    auto run = [&](auto ROLE, auto MASKED) __attribute__((always_inline)) {
+    // Steps k0 .. k1 - 1 (k0 on the 6-step grid from FS_KBEGIN).
+    auto run = [&](auto ROLE, auto MASKED, int k0, int k1) __attribute__((always_inline)) {
         static_assert((FS_KBEGIN & 1) == 0, "the unrolled steps start on an even k");
-        for (int k = FS_KBEGIN; k < KE; k += 6) {
+        for (int k = k0; k < k1; k += 6) {
             step(ROLE, MASKED, fs_ic<0>(), fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), k);
-            if (k + 1 >= KE) break;
+            if (k + 1 >= k1) break;
             step(ROLE, MASKED, fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), fs_ic<1>(), k + 1);
-            if (k + 2 >= KE) break;
+            if (k + 2 >= k1) break;
             step(ROLE, MASKED, fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), fs_ic<0>(), k + 2);
-            if (k + 3 >= KE) break;
+            if (k + 3 >= k1) break;
             step(ROLE, MASKED, fs_ic<0>(), fs_ic<1>(), fs_ic<2>(), fs_ic<1>(), k + 3);
-            if (k + 4 >= KE) break;
+            if (k + 4 >= k1) break;
             step(ROLE, MASKED, fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), fs_ic<0>(), k + 4);
-            if (k + 5 >= KE) break;
+            if (k + 5 >= k1) break;
             step(ROLE, MASKED, fs_ic<1>(), fs_ic<2>(), fs_ic<0>(), fs_ic<1>(), k + 5);
         }
     };
-    // interior workgroup: every row and column a layer keeps lies in the image
-    const bool interior = x0 - 5 >= 0 && x0 + 58 < W && Y0 - 8 >= 0 && Y1 + 8 <= H;
+    auto run_all = [&](auto ROLE) __attribute__((always_inline)) {
+        // Masking matters only where a step writes a row or column outside
+        // the image: every column of a strip at the image's side, and at the
+        // top and bottom of the image the steps whose written rows (Y0 + 2k
+        // + 3 .. Y0 + 2k + 20) leave [0, H).  The other steps run the
+        // unmasked loop (same results: nothing there is masked), so a band
+        // at the image's top or bottom spends almost all its time in the
+        // same code as an interior band.  Boundaries lie on the 6-step grid.
+        const bool cols_in = x0 - 5 >= 0 && x0 + 58 < W;
+        if (!cols_in) {
+            run(ROLE, std::true_type(), FS_KBEGIN, KE);
+            return;
+        }
+        auto grid_up = [&](int k) { return k <= FS_KBEGIN ? FS_KBEGIN : FS_KBEGIN + 6 * ((k - FS_KBEGIN + 5) / 6); };
+        auto grid_down = [&](int k) { return k <= FS_KBEGIN ? FS_KBEGIN : FS_KBEGIN + 6 * ((k - FS_KBEGIN) / 6); };
+        // first step with Y0 + 2k + 3 >= 0; first step with Y0 + 2k + 20 >= H
+        const int kt = (-3 - Y0 + 1) >> 1;
+        const int kb = (H - 20 - Y0 + 1) >> 1;
+        const int k1 = min(grid_up(kt), KE);
+        const int k2 = max(min(grid_down(kb), KE), k1);
+        if (k1 > FS_KBEGIN) run(ROLE, std::true_type(), FS_KBEGIN, k1);
+        if (k2 > k1) run(ROLE, std::false_type(), k1, k2);
+        if (KE > k2) run(ROLE, std::true_type(), k2, KE);
+    };
     const int role = wave < 4 ? R_FIRST : (wave < 6 ? R_GATHER : R_LAST);
-    if (interior) {
-        if (role == R_FIRST) run(fs_ic<R_FIRST>(), std::false_type());
-        else if (role == R_GATHER) run(fs_ic<R_GATHER>(), std::false_type());
-        else run(fs_ic<R_LAST>(), std::false_type());
-    } else {
-        if (role == R_FIRST) run(fs_ic<R_FIRST>(), std::true_type());
-        else if (role == R_GATHER) run(fs_ic<R_GATHER>(), std::true_type());
-        else run(fs_ic<R_LAST>(), std::true_type());
-    }
+    if (role == R_FIRST) run_all(fs_ic<R_FIRST>());
+    else if (role == R_GATHER) run_all(fs_ic<R_GATHER>());
+    else run_all(fs_ic<R_LAST>());
 }
 
 __global__ void zero_ints(int32_t *p, int n)
