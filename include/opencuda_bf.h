@@ -215,6 +215,11 @@ int bf_sur_train_step(bf_sur sur, const unsigned char *x, const unsigned char *y
                       float lr, float beta1, float beta2, float eps,
                       float *loss_out, void *stream);
 
+/* Loss of bf_sur_train_step and bf_sur_forward_backward: the MSE plus,
+ * per pixel, weight * max(|pred - y| - start_lsb / 255, 0)^2 (over the same
+ * pixels and count).  weight 0 (the default of a new net) is the plain MSE. */
+int bf_sur_set_loss_tail(bf_sur sur, float weight, float start_lsb);
+
 /* Forward only, FP32 output (n x h x w, device) of the same crops: the
  * gradient-check hook.  grads (optional, device, param_count floats):
  * receives d(loss)/d(params) of the loss bf_sur_train_step minimises

@@ -99,6 +99,7 @@ def lib() -> ctypes.CDLL:
         "bf_sur_get_params": ([P, P], ctypes.c_int),
         "bf_sur_set_params": ([P, P], ctypes.c_int),
         "bf_sur_train_step": ([P, P, P, P, P, I32, I32, I32, F, F, F, F, P, P], ctypes.c_int),
+        "bf_sur_set_loss_tail": ([P, F, F], ctypes.c_int),
         "bf_sur_forward_backward": ([P, P, P, P, P, I32, I32, I32, P, P, P, P], ctypes.c_int),
         "bf_sur_infer": ([P, VP, VP, F, P], ctypes.c_int),
         "bf_sur_filter": ([P, VP, VP, P, I32, I32, F, I32, P, P, P], ctypes.c_int),

@@ -717,8 +717,13 @@ __global__ void reduce_partials(const float *partial, int nparts, int psz, float
 // g6 = 2 (pred - y) / count (bf16), loss += (pred - y)^2 / count, with
 // count = N (H - 2 MARGIN) (W - 2 MARGIN) whatever the edges (the step size
 // does not depend on a batch's mix of edge crops).
+// With tail_w > 0 each pixel adds tail_w * max(|d| - tail_t, 0)^2 to the
+// loss (d = pred - y): the errors past tail_t, which are what the sampled
+// check rejects tiles for, weigh more.  tail_w = 0 leaves the MSE bits as
+// they are.
 __global__ void loss_kernel(const float *pred, const uint8_t *target, const uint8_t *edges,
-                            int N, int H, int W, __bf16 *g6, float *loss)
+                            int N, int H, int W, float tail_w, float tail_t, __bf16 *g6,
+                            float *loss)
 {
     __shared__ float red[256];
     const int64_t total = (int64_t)N * H * W;
@@ -734,8 +739,16 @@ __global__ void loss_kernel(const float *pred, const uint8_t *target, const uint
         float g = 0.0f;
         if (x >= ml && x < W - mr && y >= mt && y < H - mb) {
             const float d = pred[i] - (float)target[i] / 255.0f;
-            acc += d * d * inv;
-            g = 2.0f * d * inv;
+            float q = d * d, gd = 2.0f * d;
+            if (tail_w > 0.0f) {
+                const float a = fabsf(d) - tail_t;
+                if (a > 0.0f) {
+                    q += tail_w * a * a;
+                    gd += 2.0f * tail_w * copysignf(a, d);
+                }
+            }
+            acc += q * inv;
+            g = gd * inv;
         }
         g6[i] = to_bf(g);
     }
@@ -1363,6 +1376,7 @@ struct Net {
     float *params = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     __bf16 *fw[NLAYERS] = {}, *bw[NLAYERS] = {};
     int step = 0;
+    float tail_w = 0.0f, tail_t = 0.0f;   // loss tail term (loss_kernel)
     // activation workspace
     size_t ws_pix = 0;
     __bf16 *act[5] = {};         // layer 1..5 outputs, NHWC 32
@@ -1573,7 +1587,7 @@ int backward(Net *net, const uint8_t *img, const uint8_t *target, const uint8_t 
     const int64_t total = (int64_t)N * H * W;
     const int lb = (int)std::min<int64_t>(4096, (total + 255) / 256);
     hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, s, net->pred, target, edges, N, H, W,
-                       net->g6, loss);
+                       net->tail_w, net->tail_t, net->g6, loss);
     BF_HIP(hipGetLastError());
     int rc;
     // layer 6
@@ -1722,6 +1736,19 @@ int bf_sur_set_params(bf_sur h, const float *params)
     int rc = pack(net, nullptr);
     if (rc) return rc;
     BF_HIP(hipDeviceSynchronize());
+    return kOk;
+}
+
+int bf_sur_set_loss_tail(bf_sur h, float weight, float start_lsb)
+{
+    if (!h) return kNullPointer;
+    if (!(weight >= 0.0f && weight < 1e30f) || !(start_lsb >= 0.0f && start_lsb < 256.0f)) {
+        set_error("tail weight must be finite and >= 0, tail start in [0, 256) LSB");
+        return kInvalidData;
+    }
+    Net *net = net_of(h);
+    net->tail_w = weight;
+    net->tail_t = start_lsb / 255.0f;
     return kOk;
 }
 

@@ -71,6 +71,11 @@ class Surrogate:
                                        ctypes.c_void_p(sigma.data_ptr()), ep, n, h, w, lr, beta1,
                                        beta2, eps, lp, _stream(stream)), "bf_sur_train_step")
 
+    def set_loss_tail(self, weight: float, start_lsb: float) -> None:
+        """Loss of train_step and forward_backward: MSE + weight *
+        max(|pred - y| - start_lsb/255, 0)^2 per pixel (weight 0: MSE)."""
+        _check(lib().bf_sur_set_loss_tail(self.handle, weight, start_lsb), "bf_sur_set_loss_tail")
+
     def forward_backward(self, x, y, sigma, stream=None, edges=None):
         """Returns (pred float32 (n,h,w), grads float32 (nparams), loss) tensors."""
         import torch
@@ -271,6 +276,9 @@ def _main() -> int:
                     help="fraction of training images with the 8192^2 benchmark image's settings")
     ap.add_argument("--data-seed", type=int, default=2024, help="seed of the training images")
     ap.add_argument("--crop-seed", type=int, default=7, help="seed of the crop sequence")
+    ap.add_argument("--tail-weight", type=float, default=0.0,
+                    help="weight of the loss term max(|err| - tail-start, 0)^2 (0: plain MSE)")
+    ap.add_argument("--tail-start", type=float, default=2.0, help="start of that term, in LSB")
     args = ap.parse_args()
 
     def log(m):
@@ -284,13 +292,16 @@ def _main() -> int:
     if args.init:
         init_steps = int(net.load(args.init).get("steps", 0))
         log(f"starting from {args.init} ({init_steps} steps)")
+    if args.tail_weight > 0:
+        net.set_loss_tail(args.tail_weight, args.tail_start)
     hist = train(net, data, args.steps, args.batch, args.crop, args.lr, seed=args.crop_seed,
                  log=log, cosine=args.cosine, log_every=args.log_every, edge_frac=args.edge_frac)
     net.save(args.out, history=np.array(hist, np.float64), mode=mode, radius=15,
              sigma_space=10.0, images=args.images, steps=init_steps + args.steps, batch=args.batch,
              crop=args.crop, lr=args.lr, seed=args.seed, cosine=int(args.cosine),
              init_steps=init_steps, edge_frac=args.edge_frac, c3_frac=args.c3_frac,
-             data_seed=args.data_seed, crop_seed=args.crop_seed)
+             data_seed=args.data_seed, crop_seed=args.crop_seed, tail_weight=args.tail_weight,
+             tail_start=args.tail_start)
     log(f"saved {args.out}")
     if args.eval:
         from . import workloads

@@ -42,11 +42,12 @@ def _unpack(p):
     return out
 
 
-def _reference(params, x_u8, y_u8, sigma, margin=6, edges=None):
+def _reference(params, x_u8, y_u8, sigma, margin=6, edges=None, tail=(0.0, 0.0)):
     """float64 CPU network with the GPU's BF16 rounding points; the loss
     leaves out `margin` pixels along each crop side except the sides that
     `edges[n]` marks as image edges (bit 0 top, 1 bottom, 2 left, 3 right),
-    normalised by the interior pixel count."""
+    normalised by the interior pixel count.  tail = (weight, start in LSB)
+    adds weight * max(|d| - start/255, 0)^2 per pixel."""
     import torch
     import torch.nn.functional as F
 
@@ -85,7 +86,8 @@ def _reference(params, x_u8, y_u8, sigma, margin=6, edges=None):
         le, r = (0 if e & 4 else margin), (0 if e & 8 else margin)
         mask[i, t:h - b, le:w - r] = 1.0
     d = (pred - y) * mask
-    loss = (d * d).sum() / (n * (h - 2 * margin) * (w - 2 * margin))
+    over = torch.clamp(d.abs() - tail[1] / 255.0, min=0.0)
+    loss = ((d * d).sum() + tail[0] * (over * over).sum()) / (n * (h - 2 * margin) * (w - 2 * margin))
     loss.backward()
     grads = []
     for (cin, cout), wp, bp in zip(LAYERS, ws, bs):
@@ -98,8 +100,8 @@ def test_param_count(device):
     assert oc.lib().bf_sur_param_count() == 37889
 
 
-@pytest.mark.parametrize("edges", [None, (1 | 4, 2 | 8)])
-def test_forward_and_gradients_match_cpu_float64(device, edges):
+@pytest.mark.parametrize("edges,tail", [(None, None), ((1 | 4, 2 | 8), None), (None, (6.0, 3.0))])
+def test_forward_and_gradients_match_cpu_float64(device, edges, tail):
     import torch
     rng = np.random.default_rng(0)
     net = sg.Surrogate(seed=3)
@@ -113,10 +115,16 @@ def test_forward_and_gradients_match_cpu_float64(device, edges):
     tx, ty = torch.from_numpy(x).cuda(), torch.from_numpy(y).cuda()
     ts = torch.from_numpy(sigma).cuda()
     te = torch.tensor(edges, dtype=torch.uint8).cuda() if edges is not None else None
+    if tail is not None:
+        net.set_loss_tail(*tail)
     pred, grads, loss = net.forward_backward(tx, ty, ts, edges=te)
     torch.cuda.synchronize()
     pred, grads, loss = pred.cpu().numpy(), grads.cpu().numpy(), float(loss.item())
-    rpred, rgrads, rloss = _reference(p, x, y, sigma, edges=edges)
+    rpred, rgrads, rloss = _reference(p, x, y, sigma, edges=edges, tail=tail or (0.0, 0.0))
+    if tail is not None:
+        # the term is active on a good share of the pixels and moves the loss
+        _, _, mse = _reference(p, x, y, sigma, edges=edges)
+        assert rloss > 1.2 * mse
     assert np.abs(pred - rpred).max() < 2e-3
     assert abs(loss - rloss) / rloss < 1e-2
     o = 0
@@ -127,6 +135,15 @@ def test_forward_and_gradients_match_cpu_float64(device, edges):
             rel = np.linalg.norm(g - r) / max(np.linalg.norm(r), 1e-30)
             cos = float(np.dot(g, r) / max(np.linalg.norm(g) * np.linalg.norm(r), 1e-30))
             assert rel < 0.05 and cos > 0.998, f"layer {l + 1} {part}: rel {rel:.4f} cos {cos:.5f}"
+
+
+def test_loss_tail_arguments_are_checked(device):
+    net = sg.Surrogate(seed=3)
+    for bad in ((-1.0, 2.0), (float("nan"), 2.0), (float("inf"), 2.0), (1.0, -0.5), (1.0, 300.0)):
+        with pytest.raises(oc.BfError):
+            net.set_loss_tail(*bad)
+    net.set_loss_tail(0.0, 0.0)
+    net.close()
 
 
 def _small_data(n=48, size=128, seed=5):
