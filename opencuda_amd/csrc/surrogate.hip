@@ -129,7 +129,7 @@ __device__ __forceinline__ uint32_t relu_keep(uint32_t v, uint32_t act)
 // Workgroup = 4 waves = 64 x 4 output pixels; wave w owns row y0 + w as 4
 // M-tiles of 16 pixels.
 template <int KIND>
-__global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) conv_kernel(ConvArgs a)
 {
     using T = KTraits<KIND>;
     // Staged input tile; after the MFMAs it also holds the output tile (the
@@ -137,6 +137,7 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
     __shared__ __align__(16) __bf16 s_in[T::STAGE32 ? (TY + 2) * (TX + 2) * PITCH
                                                    : (KIND == K_LAST ? 8 : TY * TX * PITCH)];
     __shared__ float s_first[(KIND == K_FIRST || KIND == K_DG_LAST) ? (TY + 2) * (TX + 2) : 1];
+    __shared__ float s_last[KIND == K_LAST ? TY * TX : 1];   // the last layer's row sums
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int H = a.H, W = a.W;
@@ -158,6 +159,31 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
                                        : *reinterpret_cast<const bf16x8 *>(w + 8 * (lane >> 4));
         }
 
+    // 32-channel inputs: the tile's (TY+2) x (TX+2) pixels as 16-byte
+    // chunks, SE per thread, loaded into registers one tile ahead so the
+    // loads are in flight during the previous tile's MFMAs (a loop that
+    // stored each chunk as it arrived waited for every load in turn).
+    constexpr int SN = (TY + 2) * (TX + 2) * 4;
+    constexpr int SE = T::STAGE32 ? (SN + 255) / 256 : 1;
+    uint4 pre[SE];
+    auto load_stage = [&](int64_t t) {
+        const int tn = (int)(t / ((int64_t)tiles_x * tiles_y));
+        const int tr = (int)(t - (int64_t)tn * tiles_x * tiles_y);
+        const int ty0 = (tr / tiles_x) * TY, tx0 = (tr % tiles_x) * TX;
+        const __bf16 *src = a.in + (size_t)tn * npix * C;
+#pragma unroll
+        for (int k = 0; k < SE; k++) {
+            const int e = tid + 256 * k;
+            const int chunk = e & 3, pix = e >> 2;
+            const int r = pix / (TX + 2), c = pix - r * (TX + 2);
+            const int gy = ty0 - 1 + r, gx = tx0 - 1 + c;
+            pre[k] = make_uint4(0, 0, 0, 0);
+            if (e < SN && gy >= 0 && gy < H && gx >= 0 && gx < W)
+                pre[k] = *reinterpret_cast<const uint4 *>(src + ((size_t)gy * W + gx) * C + chunk * 8);
+        }
+    };
+    if (T::STAGE32 && (int64_t)blockIdx.x < ntiles) load_stage(blockIdx.x);
+
     for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
     const int n = (int)(tile / ((int64_t)tiles_x * tiles_y));
     const int trem = (int)(tile - (int64_t)n * tiles_x * tiles_y);
@@ -166,15 +192,11 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
 
     // ---- stage the input tile (rows y0-1 .. y0+4, cols x0-1 .. x0+64), zero outside
     if (T::STAGE32) {
-        const __bf16 *src = a.in + (size_t)n * npix * C;
-        for (int e = tid; e < (TY + 2) * (TX + 2) * 4; e += 256) {
-            const int chunk = e & 3, pix = e >> 2;
-            const int r = pix / (TX + 2), c = pix - r * (TX + 2);
-            const int gy = y0 - 1 + r, gx = x0 - 1 + c;
-            uint4 v = make_uint4(0, 0, 0, 0);
-            if (gy >= 0 && gy < H && gx >= 0 && gx < W)
-                v = *reinterpret_cast<const uint4 *>(src + ((size_t)gy * W + gx) * C + chunk * 8);
-            *reinterpret_cast<uint4 *>(&s_in[pix * PITCH + chunk * 8]) = v;
+#pragma unroll
+        for (int k = 0; k < SE; k++) {
+            const int e = tid + 256 * k;
+            if (e < SN)
+                *reinterpret_cast<uint4 *>(&s_in[(e >> 2) * PITCH + (e & 3) * 8]) = pre[k];
         }
     } else if (KIND == K_FIRST) {
         const uint8_t *im = a.img + (size_t)n * a.img_stride;
@@ -196,6 +218,23 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
         }
     }
     __syncthreads();
+    if (T::STAGE32 && tile + gridDim.x < ntiles) load_stage(tile + gridDim.x);
+    // data-gradient kinds: the ReLU mask of this wave's output row, loaded
+    // now so it arrives during the MFMAs (pixels 16 j + lane / 4, 16-byte
+    // chunk lane % 4, as the epilogue stores them)
+    constexpr bool DG = KIND == K_DG_HIDDEN || KIND == K_DG_LAST;
+    uint4 mask[DG ? 4 : 1];
+    if (DG) {
+        const int my = y0 + wave;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int mx = x0 + 16 * j + (lane >> 2);
+            mask[j] = make_uint4(0, 0, 0, 0);
+            if (my < H && mx < W)
+                mask[j] = *reinterpret_cast<const uint4 *>(
+                    a.mask + ((size_t)n * npix + (size_t)my * W + mx) * C + (lane & 3) * 8);
+        }
+    }
 
     const int i16 = lane & 15, kq = lane >> 4;
     // The forward layers start from their bias (the accumulator's initial
@@ -222,6 +261,11 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
                 const f32x4 p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bw[s][0], (f32x4){ 0.f, 0.f, 0.f, 0.f }, 0, 0, 0);
                 acc[m][0] = s == 0 ? p : acc[m][0] + p;
             }
+            // Finish each tap's sums before the next tap's loads: without
+            // this fence the compiler issued all 36 operand loads and
+            // products first (480 registers, one wave per SIMD).  The empty
+            // asm takes the sums and clobbers memory; it emits nothing.
+            asm volatile("" : "+v"(acc[0][0]), "+v"(acc[1][0]), "+v"(acc[2][0]), "+v"(acc[3][0])::"memory");
         }
     } else if (T::STAGE32) {
 #pragma unroll
@@ -271,30 +315,34 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
     // ---- epilogue
     const int y = y0 + wave;
     if constexpr (KIND == K_LAST) {
-        if (y < H) {
+        // Column 0 of the products (lanes i16 == 0) holds the row's 64 sums;
+        // through LDS each lane takes one pixel, so the image loads and the
+        // stores are one coalesced access per row instead of sixteen
+        // four-lane ones, each waiting for the last.
+        if (i16 == 0) {
 #pragma unroll
-            for (int m = 0; m < 4; m++) {
+            for (int m = 0; m < 4; m++)
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int x = x0 + 16 * m + 4 * kq + r;
-                    if (x >= W || i16 != 0) continue;
-                    const size_t p = (size_t)n * npix + (size_t)y * W + x;
-                    const uint8_t u = a.img[(size_t)n * a.img_stride + (int64_t)y * a.img_pitch + x];
-                    const float v = (float)u / 255.0f + (acc[m][0][r] + a.bias[0]);
-                    if (a.pred) a.pred[p] = v;
-                    if (a.out_u8) {
-                        const float q = rintf(v * 255.0f);
-                        a.out_u8[(int64_t)y * a.u8_pitch + x] =
-                            (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
-                    }
-                }
+                for (int r = 0; r < 4; r++) s_last[wave * TX + 16 * m + 4 * kq + r] = acc[m][0][r];
+        }
+        __syncthreads();
+        const int x = x0 + lane;
+        if (y < H && x < W) {
+            const size_t p = (size_t)n * npix + (size_t)y * W + x;
+            const uint8_t u = a.img[(size_t)n * a.img_stride + (int64_t)y * a.img_pitch + x];
+            const float v = (float)u / 255.0f + (s_last[wave * TX + lane] + a.bias[0]);
+            if (a.pred) a.pred[p] = v;
+            if (a.out_u8) {
+                const float q = rintf(v * 255.0f);
+                a.out_u8[(int64_t)y * a.u8_pitch + x] =
+                    (uint8_t)(q < 0.0f ? 0.0f : (q > 255.0f ? 255.0f : q));
             }
         }
     } else {
         // NHWC-32 output: each wave writes its 64 x 32 tile to LDS (the
         // staged input is dead once every wave has done its MFMAs), then
         // stores it as 16 contiguous bytes per lane, the data-gradient kinds
-        // reading the ReLU mask the same way.
+        // applying the ReLU mask loaded before the MFMAs.
         __syncthreads();
         __bf16 *s_out = s_in + wave * TX * PITCH;
 #pragma unroll
@@ -317,12 +365,11 @@ __global__ void __launch_bounds__(256) conv_kernel(ConvArgs a)
                 if (x >= W) continue;
                 const size_t p = (size_t)n * npix + (size_t)y * W + x;
                 uint4 v = *reinterpret_cast<const uint4 *>(&s_out[px * PITCH + chunk * 8]);
-                if (KIND == K_DG_HIDDEN || KIND == K_DG_LAST) {
-                    const uint4 mk = *reinterpret_cast<const uint4 *>(a.mask + p * C + chunk * 8);
-                    v.x = relu_keep(v.x, mk.x);
-                    v.y = relu_keep(v.y, mk.y);
-                    v.z = relu_keep(v.z, mk.z);
-                    v.w = relu_keep(v.w, mk.w);
+                if (DG) {
+                    v.x = relu_keep(v.x, mask[j].x);
+                    v.y = relu_keep(v.y, mask[j].y);
+                    v.z = relu_keep(v.z, mask[j].z);
+                    v.w = relu_keep(v.w, mask[j].w);
                 }
                 *reinterpret_cast<uint4 *>(a.out + p * C + chunk * 8) = v;
             }
@@ -342,22 +389,26 @@ struct WgradArgs {
     int N, H, W;
 };
 
+// Weight gradients of the first layer (inputs: the image and sigma_range):
 // dW[t][ci][co] = sum_p in[p + off(t)][ci] * d[p][co]; db[co] = sum_p d[p][co].
-// GEMM with M = (t, ci), N = co, K = pixels: a workgroup walks segments of
-// 64 pixels of one row; the input rows are staged transposed ([ci][x]) in
-// three copies shifted by dx = -1, 0, 1 so every operand is an aligned
-// 16-byte LDS read.
+// GEMM with M = (t, ci) (18 rows, two M-tiles), N = co, K = pixels: a
+// workgroup walks segments of 64 pixels of one row; the input rows are
+// staged transposed ([ci][x]) in three copies shifted by dx = -1, 0, 1 so
+// every operand is an aligned 16-byte LDS read.  Each thread's global loads
+// of a segment (one image byte, one 16-byte gradient chunk) are issued one
+// segment ahead, during the previous segment's MFMAs.  (The 32-channel
+// inputs go through wgrad32_kernel.)
 template <int KIND>
 __global__ void __launch_bounds__(256) wgrad_kernel(WgradArgs a)
 {
-    constexpr int CIN = (KIND == K_FIRST) ? 2 : C;
-    constexpr int COUT = (KIND == K_LAST) ? 1 : C;
+    static_assert(KIND == K_FIRST, "first layer only");
+    constexpr int CIN = 2, COUT = C;
     constexpr int M = 9 * CIN;
     constexpr int MT = (M + 15) / 16;
-    constexpr int NT = (COUT + 15) / 16;
-    constexpr int PAIRS = MT * NT;
-    constexpr int PPW = (PAIRS + 3) / 4;          // (m,n) tiles per wave
+    constexpr int NT = COUT / 16;
     constexpr int PSZ = 9 * CIN * COUT + COUT;
+    static_assert(MT * NT == 4, "one (m, n) tile pair per wave");
+    static_assert(3 * (TX + 2) <= 256 && TX * 4 == 256, "one staging element per thread");
     __shared__ __align__(16) __bf16 s_in[3][3][CIN][TX];
     __shared__ __align__(16) __bf16 s_d[NT * 16][TX];
 
@@ -367,111 +418,78 @@ __global__ void __launch_bounds__(256) wgrad_kernel(WgradArgs a)
     const int64_t nseg = (int64_t)a.N * a.H * segs_x;
     const int64_t npix = (int64_t)a.H * a.W;
 
-    f32x4 acc[PPW];
-#pragma unroll
-    for (int q = 0; q < PPW; q++) acc[q] = (f32x4){ 0.f, 0.f, 0.f, 0.f };
-    float dbias = 0.0f;
-
-    for (int64_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    // staging tasks: context pixel (row ir, column ic of the 3 x (TX + 2)
+    // window) for threads < 3 (TX + 2); gradient chunk dch of pixel dxx
+    const bool img_task = tid < 3 * (TX + 2);
+    const int ir = tid / (TX + 2), ic = tid - ir * (TX + 2);
+    const int dch = tid & 3, dxx = tid >> 2;
+    float pimg = -1.0f, psr = 0.0f;     // pimg < 0: outside the crop
+    uint4 pd = make_uint4(0, 0, 0, 0);
+    auto load_seg = [&](int64_t seg) {
         const int n = (int)(seg / ((int64_t)a.H * segs_x));
         const int rem = (int)(seg - (int64_t)n * a.H * segs_x);
         const int y = rem / segs_x, x0 = (rem - y * segs_x) * TX;
-        __syncthreads();
-        // stage input rows y-1..y+1, cols x0-1..x0+64
-        if (KIND == K_FIRST) {
-            const float sr = a.sigma[n] / 100.0f;
-            for (int e = tid; e < 3 * (TX + 2); e += 256) {
-                const int r = e / (TX + 2), c = e - r * (TX + 2);
-                const int gy = y - 1 + r, gx = x0 - 1 + c;
-                const bool in = gy >= 0 && gy < a.H && gx >= 0 && gx < a.W;
-                const float v0 = in ? (float)a.img[(size_t)n * npix + (size_t)gy * a.W + gx] / 255.0f : 0.0f;
-                const float v1 = in ? sr : 0.0f;
+        psr = a.sigma[n] / 100.0f;
+        pimg = -1.0f;
+        if (img_task) {
+            const int gy = y - 1 + ir, gx = x0 - 1 + ic;
+            if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
+                pimg = (float)a.img[(size_t)n * npix + (size_t)gy * a.W + gx];
+        }
+        pd = make_uint4(0, 0, 0, 0);
+        if (x0 + dxx < a.W)
+            pd = *reinterpret_cast<const uint4 *>(a.dpre + ((size_t)n * npix + (size_t)y * a.W + x0 + dxx) * C + dch * 8);
+    };
+
+    f32x4 acc = (f32x4){ 0.f, 0.f, 0.f, 0.f };
+    float dbias = 0.0f;
+    const int mt = wave / NT, nt = wave - mt * NT;
+    const int mrow = 16 * mt + i16;          // this lane's A row
+    const int t = mrow / CIN, ci = mrow - t * CIN;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+
+    if ((int64_t)blockIdx.x < nseg) load_seg(blockIdx.x);
+    for (int64_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        __syncthreads();   // the previous segment's reads of s_in / s_d are done
+        if (img_task) {
+            const bool in = pimg >= 0.0f;
+            const float v0 = in ? pimg / 255.0f : 0.0f;
+            const float v1 = in ? psr : 0.0f;
 #pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    const int xx = c - 1 - dx;
-                    if (xx >= 0 && xx < TX) {
-                        s_in[dx + 1][r][0][xx] = to_bf(v0);
-                        s_in[dx + 1][r][1][xx] = to_bf(v1);
-                    }
-                }
-            }
-        } else {
-            for (int e = tid; e < 3 * (TX + 2) * 4; e += 256) {
-                const int chunk = e & 3, pix = e >> 2;
-                const int r = pix / (TX + 2), c = pix - r * (TX + 2);
-                const int gy = y - 1 + r, gx = x0 - 1 + c;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (gy >= 0 && gy < a.H && gx >= 0 && gx < a.W)
-                    v = *reinterpret_cast<const uint4 *>(a.act + ((size_t)n * npix + (size_t)gy * a.W + gx) * C + chunk * 8);
-                const __bf16 *vb = reinterpret_cast<const __bf16 *>(&v);
-#pragma unroll
-                for (int dx = -1; dx <= 1; dx++) {
-                    const int xx = c - 1 - dx;
-                    if (xx >= 0 && xx < TX) {
-#pragma unroll
-                        for (int q = 0; q < 8; q++) s_in[dx + 1][r][chunk * 8 + q][xx] = vb[q];
-                    }
+            for (int sx = -1; sx <= 1; sx++) {
+                const int xx = ic - 1 - sx;
+                if (xx >= 0 && xx < TX) {
+                    s_in[sx + 1][ir][0][xx] = to_bf(v0);
+                    s_in[sx + 1][ir][1][xx] = to_bf(v1);
                 }
             }
         }
-        // stage the gradient row y, cols x0..x0+63 transposed
-        if (KIND == K_LAST) {
-            for (int e = tid; e < 16 * TX; e += 256) {
-                const int co = e / TX, xx = e - co * TX;
-                const int gx = x0 + xx;
-                __bf16 v = to_bf(0.0f);
-                if (co == 0 && gx < a.W) v = a.dpre[(size_t)n * npix + (size_t)y * a.W + gx];
-                s_d[co][xx] = v;
-            }
-        } else {
-            for (int e = tid; e < TX * 4; e += 256) {
-                const int chunk = e & 3, xx = e >> 2;
-                const int gx = x0 + xx;
-                uint4 v = make_uint4(0, 0, 0, 0);
-                if (gx < a.W)
-                    v = *reinterpret_cast<const uint4 *>(a.dpre + ((size_t)n * npix + (size_t)y * a.W + gx) * C + chunk * 8);
-                const __bf16 *vb = reinterpret_cast<const __bf16 *>(&v);
+        {
+            const __bf16 *vb = reinterpret_cast<const __bf16 *>(&pd);
 #pragma unroll
-                for (int q = 0; q < 8; q++) s_d[chunk * 8 + q][xx] = vb[q];
-            }
+            for (int q = 0; q < 8; q++) s_d[dch * 8 + q][dxx] = vb[q];
         }
         __syncthreads();
+        if (seg + gridDim.x < nseg) load_seg(seg + gridDim.x);
         if (tid < COUT) {
             float s = 0.0f;
             for (int xx = 0; xx < TX; xx++) s += from_bf(s_d[tid][xx]);
             dbias += s;
         }
 #pragma unroll
-        for (int q = 0; q < PPW; q++) {
-            const int pair = wave * PPW + q;
-            if (pair >= PAIRS) break;
-            const int mt = pair / NT, nt = pair - mt * NT;
-            const int mrow = 16 * mt + i16;          // this lane's A row
-            const int t = mrow / CIN, ci = mrow - t * CIN;
-#pragma unroll
-            for (int ks = 0; ks < TX / 32; ks++) {
-                bf16x8 av = (bf16x8){ 0, 0, 0, 0, 0, 0, 0, 0 };
-                if (mrow < M) {
-                    const int dy = t / 3 - 1, dx = t % 3 - 1;
-                    av = *reinterpret_cast<const bf16x8 *>(&s_in[dx + 1][dy + 1][ci][32 * ks + 8 * kq]);
-                }
-                const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&s_d[16 * nt + i16][32 * ks + 8 * kq]);
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc[q], 0, 0, 0);
-            }
+        for (int ks = 0; ks < TX / 32; ks++) {
+            bf16x8 av = (bf16x8){ 0, 0, 0, 0, 0, 0, 0, 0 };
+            if (mrow < M) av = *reinterpret_cast<const bf16x8 *>(&s_in[dx + 1][dy + 1][ci][32 * ks + 8 * kq]);
+            const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&s_d[16 * nt + i16][32 * ks + 8 * kq]);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
         }
     }
     // write this workgroup's partial sums
     float *part = a.partial + (size_t)blockIdx.x * PSZ;
 #pragma unroll
-    for (int q = 0; q < PPW; q++) {
-        const int pair = wave * PPW + q;
-        if (pair >= PAIRS) break;
-        const int mt = pair / NT, nt = pair - mt * NT;
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = 16 * mt + 4 * kq + r, co = 16 * nt + i16;
-            if (m < M && co < COUT) part[m * COUT + co] = acc[q][r];
-        }
+    for (int r = 0; r < 4; r++) {
+        const int m = 16 * mt + 4 * kq + r, co = 16 * nt + i16;
+        if (m < M) part[m * COUT + co] = acc[r];
     }
     if (tid < COUT) part[9 * CIN * COUT + tid] = dbias;
 }
@@ -726,31 +744,36 @@ __global__ void loss_kernel(const float *pred, const uint8_t *target, const uint
                             float *loss)
 {
     __shared__ float red[256];
-    const int64_t total = (int64_t)N * H * W;
+    const int64_t rows = (int64_t)N * H;
     const int64_t count = (int64_t)N * (H - 2 * MARGIN) * (W - 2 * MARGIN);
     const float inv = 1.0f / (float)count;
     float acc = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int x = (int)(i % W), y = (int)((i / W) % H);
-        const int e = edges ? edges[i / ((int64_t)H * W)] : 0;
+    // a workgroup per image row at a time (its crop, row and margins set
+    // once per row), its threads along the row
+    for (int64_t row = blockIdx.x; row < rows; row += gridDim.x) {
+        const int n = (int)(row / H), y = (int)(row - (int64_t)n * H);
+        const int e = edges ? edges[n] : 0;
         const int mt = (e & 1) ? 0 : MARGIN, mb = (e & 2) ? 0 : MARGIN;
         const int ml = (e & 4) ? 0 : MARGIN, mr = (e & 8) ? 0 : MARGIN;
-        float g = 0.0f;
-        if (x >= ml && x < W - mr && y >= mt && y < H - mb) {
-            const float d = pred[i] - (float)target[i] / 255.0f;
-            float q = d * d, gd = 2.0f * d;
-            if (tail_w > 0.0f) {
-                const float a = fabsf(d) - tail_t;
-                if (a > 0.0f) {
-                    q += tail_w * a * a;
-                    gd += 2.0f * tail_w * copysignf(a, d);
+        const bool yin = y >= mt && y < H - mb;
+        for (int x = threadIdx.x; x < W; x += blockDim.x) {
+            const int64_t i = row * W + x;
+            float g = 0.0f;
+            if (yin && x >= ml && x < W - mr) {
+                const float d = pred[i] - (float)target[i] / 255.0f;
+                float q = d * d, gd = 2.0f * d;
+                if (tail_w > 0.0f) {
+                    const float a = fabsf(d) - tail_t;
+                    if (a > 0.0f) {
+                        q += tail_w * a * a;
+                        gd += 2.0f * tail_w * copysignf(a, d);
+                    }
                 }
+                acc += q * inv;
+                g = gd * inv;
             }
-            acc += q * inv;
-            g = gd * inv;
+            g6[i] = to_bf(g);
         }
-        g6[i] = to_bf(g);
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -1584,8 +1607,9 @@ int launch_wgrad(Net *net, const WgradArgs &w0, int layer, hipStream_t s)
 int backward(Net *net, const uint8_t *img, const uint8_t *target, const uint8_t *edges,
              const float *sigma, int N, int H, int W, float *loss, hipStream_t s)
 {
-    const int64_t total = (int64_t)N * H * W;
-    const int lb = (int)std::min<int64_t>(4096, (total + 255) / 256);
+    // 512 workgroups: each adds its partial sum to the one loss float, and
+    // those atomics serialise
+    const int lb = (int)std::min<int64_t>(512, (int64_t)N * H);
     hipLaunchKernelGGL(loss_kernel, dim3(lb), dim3(256), 0, s, net->pred, target, edges, N, H, W,
                        net->tail_w, net->tail_t, net->g6, loss);
     BF_HIP(hipGetLastError());

@@ -221,8 +221,11 @@ def train(net: Surrogate, data: Dataset, steps: int = 1500, batch: int = 64, cro
     for s in range(1, steps + 1):
         x, y, sg, edges = data.batch(rng, batch, crop, edge_frac)
         lr_s = lr * 0.5 * (1.0 + math.cos(math.pi * (s - 1) / steps)) if cosine else lr
-        net.train_step(x, y, sg, lr=lr_s, loss=loss, edges=edges if edge_frac > 0 else None)
-        if s % log_every == 0 or s == 1 or s == steps:
+        logged = s % log_every == 0 or s == 1 or s == steps
+        # the loss is summed (atomics on one float) only on the logged steps
+        net.train_step(x, y, sg, lr=lr_s, loss=loss if logged else None,
+                       edges=edges if edge_frac > 0 else None)
+        if logged:
             v = float(loss.item())
             hist.append((s, v))
             if log:
