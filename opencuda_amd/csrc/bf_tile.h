@@ -247,11 +247,12 @@ constexpr int ring_min_offset()
 }
 
 // Taps scheduled together between fences in the unrolled kernels.
-// (Measured on MI355X, c3 image: REF is flat from 3 to 8 taps -- its VALU
-// is ~94% busy -- while STD gains 8-14% going from 2 taps to 12 at r = 15
-// and 24 at r = 3..9; at r = 15, 16 and 24 taps measured slower than 12.)
+// (Measured on MI355X, c3 image, alternating builds: REF is flat from 4 to
+// 8 taps and slower at 2; STD gains 8-14% going from 2 taps to 12 at r = 15
+// and 24 at r = 3..9, and at r = 15 runs 8.90 ms at 20 taps against 9.14
+// at 12, 9.02 at 16, 9.23 at 18, 9.20 at 22 and 9.13 at 24 or 32.)
 template <int MODE, int R>
-constexpr int ring_group() { return MODE == kModeRef ? 4 : (R >= 13 ? 12 : 24); }
+constexpr int ring_group() { return MODE == kModeRef ? 4 : (R >= 13 ? 20 : 24); }
 
 template <class F, int... I>
 __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>)
