@@ -327,14 +327,16 @@ __global__ void __launch_bounds__(64) check_kernel(CheckArgs a)
 }
 
 // The canonical ring template, unrolled: a persistent workgroup of four
-// waves stages the range table E[d] and the per-q weights g once, then each
-// wave takes one tile at a time, stages its context as bytes in its own LDS
-// window and runs the compiled tap list for its 64 samples (tap offsets and
-// weight rows are immediates, as in bf_ring_kernel).  The factor is formed
-// per tap as g * E[d] -- the product the fused tables hold, with the same
-// single rounding -- so LDS holds 4 windows + 1 KB + the weights, and
-// several workgroups share a CU (the taps' LDS latency needs the waves).
-constexpr int kCheckGroup = 8;            // taps per scheduling window (8, 16, 32 measured alike)
+// waves stages the range table E[d] once, then each wave takes one tile at
+// a time, stages its context as bytes in its own LDS window and runs the
+// compiled tap list for its 64 samples (tap offsets and weight rows are
+// immediates, as in bf_ring_kernel).  The factor is formed per tap as
+// g * E[d] -- the product the fused tables hold, with the same single
+// rounding -- with g, the same for every lane, read from the job's weight
+// row by a scalar load (an LDS copy cost one more LDS read per tap: 0.40 ms
+// against 0.37 on c3).  LDS holds 4 windows + 1 KB, so several workgroups
+// share a CU (the taps' LDS latency needs the waves).
+constexpr int kCheckGroup = 8;            // taps per scheduling window (4 to 32 measured alike)
 
 template <int R, int MODE, int HALO>
 __global__ void __launch_bounds__(256) ring_check_kernel(CheckArgs a, int ntiles)
@@ -342,14 +344,12 @@ __global__ void __launch_bounds__(256) ring_check_kernel(CheckArgs a, int ntiles
     constexpr int SPAN = 64 + 2 * HALO;
     constexpr uint32_t WIN = SPAN * SPAN;
     constexpr uint32_t E_BASE = kTileOff + 4 * WIN;
-    constexpr uint32_t G_BASE = E_BASE + 1024;
     constexpr int T = RingTaps<R, MODE>::T;
     constexpr int MINB = ring_min_offset<R, MODE, SPAN>();
     const DevJob &j = a.job;
     const int tid = threadIdx.x, wave = tid >> 6, cell = tid & 63;
     const uint32_t win = kTileOff + (uint32_t)wave * WIN;
     for (int d = tid; d < 256; d += 256) lds_store_u32(E_BASE + 4 * d, __float_as_uint(j.range[d]));
-    for (int r = tid; r < j.nrows; r += 256) lds_store_u32(G_BASE + 4 * r, __float_as_uint(j.rowg[r]));
     const int groups = (ntiles + 3) / 4;
     for (int g = blockIdx.x; g < groups; g += gridDim.x) {
         const int tile = 4 * g + wave;
@@ -377,7 +377,7 @@ __global__ void __launch_bounds__(256) ring_check_kernel(CheckArgs a, int ntiles
                 // v_perm_b32 with selector 0 replicates byte 0 (one full-rate op)
                 const uint32_t p4 = __builtin_amdgcn_perm(p, p, 0u);
                 const float e = lds_f32(__builtin_amdgcn_sad_u8(p4, c4, E_BASE));
-                const float f = lds_f32(G_BASE + 4u * (uint32_t)tp.row) * e;
+                const float f = j.rowg[tp.row] * e;
                 t = t + f * (float)(p4 & 0xFFu);
                 s = s + f;
                 if constexpr (decltype(ci)::value % kCheckGroup == kCheckGroup - 1)
@@ -546,8 +546,7 @@ template <int R, int MODE, int HALO>
 int launch_ring_check_one(const CheckArgs &a, int ntiles, hipStream_t s)
 {
     constexpr auto kern = ring_check_kernel<R, MODE, HALO>;
-    const size_t lds = kTileOff + 4 * (size_t)(64 + 2 * HALO) * (64 + 2 * HALO) + 1024 +
-                       4 * (size_t)a.job.nrows;
+    const size_t lds = kTileOff + 4 * (size_t)(64 + 2 * HALO) * (64 + 2 * HALO) + 1024;
     BF_HIP(allow_full_lds<kern>());
     const int res = resident_groups(reinterpret_cast<const void *>(kern), 256, lds);
     if (res < 0) return hip_fail(hipGetLastError(), "occupancy query");
