@@ -175,12 +175,27 @@ __device__ __forceinline__ void build_lut(const DevJob &job, uint32_t lut_base, 
             lds_store_u32(lut_base + (uint32_t)d * 4, __float_as_uint(job.range[d]));
         return;
     }
-    const int n = (job.nrows + 1) * 256;
-    for (int e = tid; e < n; e += NT) {
-        const int row = e >> 8, d = e & 255;
-        float w = 0.0f;
-        if (row < job.nrows) w = job.rowg[row] * job.range[d];
-        lds_store_u32(lut_base + (uint32_t)e * 4, __float_as_uint(w));
+    if constexpr (NT % 256 == 0) {
+        // Each thread keeps one column d (its range value loaded once); the
+        // row a wave writes is the same for all its lanes, so its spatial
+        // weight is a scalar load.  (A loop over flat indices waited for
+        // two vector loads per entry, ~30 round trips before the first
+        // tile of a launch or of a job with other tables.)
+        const int d = tid & 255;
+        const float rd = job.range[d];
+        const int r0 = __builtin_amdgcn_readfirstlane(tid >> 8);
+        for (int row = r0; row <= job.nrows; row += NT / 256) {
+            const float w = row < job.nrows ? job.rowg[row] * rd : 0.0f;
+            lds_store_u32(lut_base + (uint32_t)(row * 256 + d) * 4, __float_as_uint(w));
+        }
+    } else {
+        const int n = (job.nrows + 1) * 256;
+        for (int e = tid; e < n; e += NT) {
+            const int row = e >> 8, d = e & 255;
+            float w = 0.0f;
+            if (row < job.nrows) w = job.rowg[row] * job.range[d];
+            lds_store_u32(lut_base + (uint32_t)e * 4, __float_as_uint(w));
+        }
     }
 }
 
